@@ -1,0 +1,186 @@
+/*
+ * mustard_hip.h -- C ABI of libmustard_hip.so, the MI355X (gfx950) tiled LU /
+ * Cholesky engine.
+ *
+ * Plain C types only: device pointers are `double *`, HIP streams and events
+ * are `void *` (a hipStream_t / hipEvent_t), handles are opaque.  Every call
+ * returns 0 on success or a negative MU_E* code; mu_last_error() gives the
+ * message of the last failure on the calling thread.  Nothing here calls
+ * exit().
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the ParCoreLab/mustard tree).
+ */
+#ifndef MUSTARD_HIP_H
+#define MUSTARD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MU_OK 0
+#define MU_EINVAL (-1)  /* bad argument */
+#define MU_EHIP (-2)    /* HIP runtime error */
+#define MU_ESTATE (-3)  /* call out of order (e.g. end without begin) */
+#define MU_ERUN (-4)    /* device run failed (scheduler time-out) */
+#define MU_ENOTSUP (-5) /* combination not supported */
+
+/* Execution modes of a recorded DAG. */
+#define MU_MODE_TILED 0    /* one hipGraph, one kernel node per unit (reference --tiled) */
+#define MU_MODE_SUBGRAPH 1 /* one persistent scheduler kernel (reference --subgraph) */
+#define MU_MODE_STREAM 2   /* units launched one by one on the stream (debugging) */
+
+/* Precision of trailing GEMM/SYRK updates. */
+#define MU_PREC_FP64 0
+#define MU_PREC_FP32 1 /* operands rounded to FP32, FP32 MFMA, FP32 accumulation */
+#define MU_PREC_BF16 2 /* operands rounded to BF16, FP32 accumulation */
+
+typedef struct mu_dag_s *mu_dag_t;
+typedef struct mu_exec_s *mu_exec_t;
+
+typedef struct {
+    int grid;           /* persistent grid size, 0 = one workgroup per CU slot */
+    double timeout_s;   /* persistent: abort after this long without progress (0 = 30 s) */
+    int trace;          /* persistent: record per-item wall-clock stamps */
+    int small_max;      /* diagonal factorizations above this order nest (0 = 128) */
+    int trsm_max;       /* triangular solves above this order nest (0 = 1024) */
+} mu_exec_opts_t;
+
+typedef struct {
+    int mode;
+    int grid;
+    long long units;
+    long long items;
+    long long edges;
+    double flops;             /* algorithmic flops of the recorded operations */
+    double critical_path_us;  /* scheduler's estimate */
+    int priority_units[4];    /* units per priority level */
+    double wall_clock_khz;
+} mu_exec_stats_t;
+
+/* ---- errors and devices ------------------------------------------------ */
+/* utils.h:8-18 (checkCudaErrors) becomes a status code plus this message. */
+const char *mu_last_error(void);
+int mu_get_device_count(int *count);
+/* cli.h:101-116 (initNvshmemDevice -> cudaSetDevice(myPE)); one GPU here. */
+int mu_set_device(int device);
+int mu_device_props(int device, int *cu_count, int *clock_khz, size_t *lds_per_block,
+                    char *name, int name_len);
+
+/* ---- memory, streams and events (utils.h:70-110, CudaEventClock) -------- */
+int mu_malloc(void **ptr, size_t bytes);
+int mu_free(void *ptr);
+int mu_memcpy(void *dst, const void *src, size_t bytes); /* direction inferred */
+int mu_memcpy_async(void *dst, const void *src, size_t bytes, void *stream);
+int mu_memset_zero(void *ptr, size_t bytes);
+int mu_stream_create(void **stream);
+int mu_stream_destroy(void *stream);
+int mu_stream_synchronize(void *stream);
+int mu_device_synchronize(void);
+int mu_event_create(void **event);
+int mu_event_destroy(void *event);
+int mu_event_record(void *event, void *stream);
+int mu_event_elapsed_ms(float *ms, void *start, void *end);
+/* utils.h:20-39 (showMemUsage) */
+int mu_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- tile operations ------------------------------------------------------
+ * The arithmetic the reference takes from cuBLAS / cuSOLVER.  Column-major,
+ * device pointers, LAPACK argument meaning.  When `stream` is being recorded
+ * by mu_dag_capture_begin the call is added to the current task of that DAG
+ * (the analogue of stream capture in mustard.h:155-166); otherwise it runs on
+ * `stream` at once.                                                         */
+/* cublasGemmEx(..., CUDA_R_64F, COMPUTE_64F) -- lu_mustard.cu:511-522,
+ * cholesky_mustard.cu:376-387 */
+int mu_dgemm(void *stream, char transa, char transb, int m, int n, int k, double alpha,
+             const double *A, int lda, const double *B, int ldb, double beta, double *C,
+             int ldc);
+/* cublasDsyrk -- cholesky_mustard.cu:327-333 (uplo 'L', trans 'N') */
+int mu_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha, const double *A,
+             int lda, double beta, double *C, int ldc);
+/* cublasDtrsm -- lu_mustard.cu:413-422, 459-468; cholesky_mustard.cu:281-290.
+ * All side/uplo/trans/diag combinations. */
+int mu_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+             double alpha, const double *A, int lda, double *B, int ldb);
+/* cusolverDnDgetrf with devIpiv = NULL -- lu_mustard.cu:360-369.  `info_dev`
+ * (device int, may be NULL) receives the 1-based column of the first zero
+ * pivot through an atomic minimum: initialise it to INT_MAX (mu_info_reset). */
+int mu_dgetrf_nopiv(void *stream, int n, double *A, int lda, int *info_dev);
+/* cusolverDnDpotrf(LOWER) -- cholesky_mustard.cu:238-246 */
+int mu_dpotrf(void *stream, char uplo, int n, double *A, int lda, int *info_dev);
+/* Set an info cell to INT_MAX (= no failure) on `stream`. */
+int mu_info_reset(int *info_dev, void *stream);
+/* Precision of the GEMM/SYRK calls recorded next on `stream` (MU_PREC_*). */
+int mu_set_update_precision(void *stream, int prec);
+
+/* ---- task DAG (mustard::TiledGraphCreator, mustard.h:132-384) ------------ */
+int mu_dag_create(mu_dag_t *dag);
+int mu_dag_destroy(mu_dag_t dag);
+int mu_dag_capture_begin(mu_dag_t dag, void *stream);
+int mu_dag_capture_end(mu_dag_t dag, void *stream);
+/* beginCaptureOperation(W, R) -- mustard.h:148-167.  `read_tiles` holds
+ * nread (row, col) pairs.  Returns the new task's index in *task. */
+int mu_dag_begin_task(mu_dag_t dag, int w_row, int w_col, const int *read_tiles, int nread,
+                      int *task);
+/* endCaptureOperation -- mustard.h:169-181 */
+int mu_dag_end_task(mu_dag_t dag);
+int mu_dag_num_tasks(mu_dag_t dag, int *ntasks);
+/* subgraphDependencies[task] -- mustard.h:137, 279-296 */
+int mu_dag_task_deps(mu_dag_t dag, int task, int *deps, int max, int *ndeps);
+/* insertDependencyKernel(src, dst, ...) -- mustard.h:197-211 */
+int mu_dag_add_edge(mu_dag_t dag, int src, int dst);
+/* Problem tag for batched recording: tiles of different tags never alias. */
+int mu_dag_set_tag(mu_dag_t dag, int tag);
+
+/* ---- executables ---------------------------------------------------------
+ * mu_exec_create lowers the DAG and uploads it (the reference's graph
+ * instantiation, lu_mustard.cu:543-604 / 633-636); mu_exec_launch replays it
+ * (cudaGraphLaunch of the scheduler or of the tiled graph, lu_mustard.cu:609,
+ * 640).  The recorded device pointers must stay valid. */
+int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t *exec);
+int mu_exec_launch(mu_exec_t exec, void *stream);
+/* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
+int mu_exec_check(mu_exec_t exec);
+int mu_exec_stats(mu_exec_t exec, mu_exec_stats_t *stats);
+/* Per-item (start, end) wall-clock stamps of the last persistent run;
+ * `stamps` holds 2 * items entries.  Needs opts.trace. */
+int mu_exec_trace(mu_exec_t exec, unsigned long long *stamps, size_t count);
+/* Unit of each item and task of each unit (for trace export). */
+int mu_exec_item_units(mu_exec_t exec, int *unit_of_item, int *task_of_unit, int *op_of_unit);
+int mu_exec_destroy(mu_exec_t exec);
+
+/* ---- whole tiled factorizations ------------------------------------------
+ * Record the reference's task sequence for an n x n column-major matrix cut
+ * into t x t tiles (lu_mustard.cu:343-535, cholesky_mustard.cu:221-400) into
+ * `dag`.  `prec` selects the trailing-update precision. */
+int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info_dev, int prec);
+int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int *info_dev,
+                             int prec);
+
+/* ---- generators (include/gen.h) ----------------------------------------- */
+/* gen.h:23-38: glibc srand(420) symmetric matrix plus n I, host memory. */
+void mu_gen_ref_host(double *A, size_t n);
+/* Counter-based uniform stream on the device (SplitMix64): M[k] = lo + (hi -
+ * lo) * u(seed, k). */
+int mu_gen_uniform(double *M, size_t count, uint64_t seed, double lo, double hi, void *stream);
+/* A = M M^T / n + n I with M uniform [-1, 1) from `seed`, built on the device
+ * with the library's own GEMM; `work` is an n*n device scratch buffer. */
+int mu_gen_spd(double *A, int n, int lda, uint64_t seed, double *work, void *stream);
+
+/* ---- verification (include/verify.h) ------------------------------------ */
+/* verify.h:17-71 / 79-129: sum |A - LU| (or |A - L L^T|) by the triple
+ * loop, host memory, small n. */
+double mu_verify_lu_l1_host(const double *A, const double *F, int n);
+double mu_verify_chol_l1_host(const double *A, const double *F, int n);
+/* ||A - L U||_F / (||A||_F n eps) on the device (chol != 0: L L^T).  A and F
+ * are n x n with leading dimension lda; `work` is 3 n*n doubles of scratch. */
+int mu_scaled_residual(const double *A, const double *F, int n, int lda, int chol,
+                       double *work, double *result, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,814 @@
+// mu_capi.cpp -- the extern "C" entry points of libmustard_hip.so.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mustard_hip.h"
+#include "mu_dag.hpp"
+#include "mu_runtime.hpp"
+
+struct mu_dag_s {
+    mu::Dag dag;
+};
+
+struct mu_exec_s {
+    mu::Exec *e = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPTRY(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(MU_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+// Streams being recorded into a DAG, and the update precision per stream.
+struct StreamState {
+    mu_dag_t dag = nullptr;
+    int prec = MU_PREC_FP64;
+};
+std::mutex g_mu;
+std::map<void *, StreamState> g_streams;
+
+StreamState stream_state(void *s)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_streams.find(s);
+    return it == g_streams.end() ? StreamState() : it->second;
+}
+
+int prec_flags(int prec)
+{
+    return prec == MU_PREC_FP32 ? mu::F_LOW_F32 : prec == MU_PREC_BF16 ? mu::F_LOW_BF16 : 0;
+}
+
+// Record `op` into the DAG capturing `stream`, or run it now.
+int issue(void *stream, const mu::OpRec &op)
+{
+    StreamState st = stream_state(stream);
+    if (st.dag) {
+        if (st.dag->dag.add_op(op) != 0)
+            return fail(MU_ESTATE, st.dag->dag.err);
+        return MU_OK;
+    }
+    mu::Dag d;
+    d.begin_task({0, 0}, {});
+    d.add_op(op);
+    d.end_task();
+    mu::Lowered low;
+    std::string err;
+    if (!mu::lower(d, mu::LowerOpts(), low, err))
+        return fail(MU_EINVAL, err);
+    if (mu::run_units(low, (hipStream_t)stream, err) != 0)
+        return fail(MU_EHIP, err);
+    return MU_OK;
+}
+
+bool is_t(char c) { return c == 'T' || c == 't' || c == 'C' || c == 'c'; }
+bool is_n(char c) { return c == 'N' || c == 'n'; }
+char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 'a' + 'A') : c; }
+
+}  // namespace
+
+extern "C" {
+
+const char *mu_last_error(void) { return g_err.c_str(); }
+
+int mu_get_device_count(int *count)
+{
+    if (!count)
+        return fail(MU_EINVAL, "count is NULL");
+    HIPTRY(hipGetDeviceCount(count));
+    return MU_OK;
+}
+
+int mu_set_device(int device)
+{
+    HIPTRY(hipSetDevice(device));
+    return MU_OK;
+}
+
+int mu_device_props(int device, int *cu_count, int *clock_khz, size_t *lds_per_block,
+                    char *name, int name_len)
+{
+    hipDeviceProp_t p;
+    HIPTRY(hipGetDeviceProperties(&p, device));
+    if (cu_count)
+        *cu_count = p.multiProcessorCount;
+    if (clock_khz)
+        *clock_khz = p.clockRate;
+    if (lds_per_block)
+        *lds_per_block = p.sharedMemPerBlock;
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    return MU_OK;
+}
+
+int mu_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr)
+        return fail(MU_EINVAL, "ptr is NULL");
+    HIPTRY(hipMalloc(ptr, bytes));
+    return MU_OK;
+}
+
+int mu_free(void *ptr)
+{
+    HIPTRY(hipFree(ptr));
+    return MU_OK;
+}
+
+int mu_memcpy(void *dst, const void *src, size_t bytes)
+{
+    HIPTRY(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    return MU_OK;
+}
+
+int mu_memcpy_async(void *dst, const void *src, size_t bytes, void *stream)
+{
+    HIPTRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, (hipStream_t)stream));
+    return MU_OK;
+}
+
+int mu_memset_zero(void *ptr, size_t bytes)
+{
+    HIPTRY(hipMemset(ptr, 0, bytes));
+    return MU_OK;
+}
+
+int mu_stream_create(void **stream)
+{
+    hipStream_t s;
+    HIPTRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return MU_OK;
+}
+
+int mu_stream_destroy(void *stream)
+{
+    HIPTRY(hipStreamDestroy((hipStream_t)stream));
+    return MU_OK;
+}
+
+int mu_stream_synchronize(void *stream)
+{
+    HIPTRY(hipStreamSynchronize((hipStream_t)stream));
+    return MU_OK;
+}
+
+int mu_device_synchronize(void)
+{
+    HIPTRY(hipDeviceSynchronize());
+    return MU_OK;
+}
+
+int mu_event_create(void **event)
+{
+    hipEvent_t e;
+    HIPTRY(hipEventCreate(&e));
+    *event = e;
+    return MU_OK;
+}
+
+int mu_event_destroy(void *event)
+{
+    HIPTRY(hipEventDestroy((hipEvent_t)event));
+    return MU_OK;
+}
+
+int mu_event_record(void *event, void *stream)
+{
+    HIPTRY(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return MU_OK;
+}
+
+int mu_event_elapsed_ms(float *ms, void *start, void *end)
+{
+    HIPTRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+    return MU_OK;
+}
+
+int mu_mem_info(size_t *free_bytes, size_t *total_bytes)
+{
+    HIPTRY(hipMemGetInfo(free_bytes, total_bytes));
+    return MU_OK;
+}
+
+// ---- tile operations ------------------------------------------------------
+int mu_dgemm(void *stream, char transa, char transb, int m, int n, int k, double alpha,
+             const double *A, int lda, const double *B, int ldb, double beta, double *C, int ldc)
+{
+    const bool ta = is_t(transa), tb = is_t(transb);
+    if ((!ta && !is_n(transa)) || (!tb && !is_n(transb)))
+        return fail(MU_EINVAL, "mu_dgemm: trans must be N or T");
+    if (m < 0 || n < 0 || k < 0)
+        return fail(MU_EINVAL, "mu_dgemm: negative size");
+    if (lda < std::max(1, ta ? k : m) || ldb < std::max(1, tb ? n : k) || ldc < std::max(1, m))
+        return fail(MU_EINVAL, "mu_dgemm: leading dimension too small");
+    mu::OpRec op;
+    op.op = mu::OP_GEMM;
+    op.flags = (ta ? mu::F_TA : 0) | (tb ? mu::F_TB : 0) | prec_flags(stream_state(stream).prec);
+    op.m = m;
+    op.n = n;
+    op.k = k;
+    op.alpha = alpha;
+    op.beta = beta;
+    op.a = A;
+    op.lda = lda;
+    op.b = B;
+    op.ldb = ldb;
+    op.c = C;
+    op.ldc = ldc;
+    return issue(stream, op);
+}
+
+int mu_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha, const double *A,
+             int lda, double beta, double *C, int ldc)
+{
+    if (up(uplo) != 'L' || !is_n(trans))
+        return fail(MU_ENOTSUP, "mu_dsyrk: only uplo = L, trans = N");
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n))
+        return fail(MU_EINVAL, "mu_dsyrk: bad size or leading dimension");
+    mu::OpRec op;
+    op.op = mu::OP_SYRK;
+    op.flags = prec_flags(stream_state(stream).prec);
+    op.m = op.n = n;
+    op.k = k;
+    op.alpha = alpha;
+    op.beta = beta;
+    op.a = A;
+    op.lda = lda;
+    op.c = C;
+    op.ldc = ldc;
+    return issue(stream, op);
+}
+
+int mu_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+             double alpha, const double *A, int lda, double *B, int ldb)
+{
+    side = up(side);
+    uplo = up(uplo);
+    diag = up(diag);
+    const bool t = is_t(transa);
+    if ((side != 'L' && side != 'R') || (uplo != 'L' && uplo != 'U') ||
+        (!t && !is_n(transa)) || (diag != 'U' && diag != 'N'))
+        return fail(MU_EINVAL, "mu_dtrsm: bad side/uplo/trans/diag");
+    if (m < 0 || n < 0 || ldb < std::max(1, m) || lda < std::max(1, side == 'L' ? m : n))
+        return fail(MU_EINVAL, "mu_dtrsm: bad size or leading dimension");
+    if (alpha == 0.0)
+        return fail(MU_ENOTSUP, "mu_dtrsm: alpha = 0");
+    // Reduce to T Y = alpha Z with T lower triangular:
+    //   left:  op(A) X = B      -> T = op(A),    Z = B
+    //   right: X op(A) = B      -> T = op(A)^T,  Z = B^T
+    // An upper T is handled by reversing the row order (F_REV).
+    bool tt;  // T = A^T
+    if (side == 'L')
+        tt = t;
+    else
+        tt = !t;
+    const bool t_upper = tt ? (uplo == 'L') : (uplo == 'U');
+    mu::OpRec op;
+    op.op = mu::OP_TRSM;
+    op.flags = (tt ? mu::F_T_TRANS : 0) | (t_upper ? mu::F_REV : 0) |
+               (side == 'R' ? mu::F_Z_ROWS : 0) | (diag == 'U' ? mu::F_UNIT : 0);
+    op.m = side == 'L' ? m : n;  // order of the triangle
+    op.n = side == 'L' ? n : m;  // right-hand sides
+    op.alpha = alpha;
+    op.a = A;
+    op.lda = lda;
+    op.c = B;
+    op.ldc = ldb;
+    return issue(stream, op);
+}
+
+int mu_dgetrf_nopiv(void *stream, int n, double *A, int lda, int *info_dev)
+{
+    if (n < 0 || lda < std::max(1, n))
+        return fail(MU_EINVAL, "mu_dgetrf_nopiv: bad size or leading dimension");
+    mu::OpRec op;
+    op.op = mu::OP_GETRF;
+    op.m = op.n = n;
+    op.c = A;
+    op.ldc = lda;
+    op.info = info_dev;
+    op.flags = prec_flags(stream_state(stream).prec);
+    return issue(stream, op);
+}
+
+int mu_dpotrf(void *stream, char uplo, int n, double *A, int lda, int *info_dev)
+{
+    if (up(uplo) != 'L')
+        return fail(MU_ENOTSUP, "mu_dpotrf: only uplo = L");
+    if (n < 0 || lda < std::max(1, n))
+        return fail(MU_EINVAL, "mu_dpotrf: bad size or leading dimension");
+    mu::OpRec op;
+    op.op = mu::OP_POTRF;
+    op.m = op.n = n;
+    op.c = A;
+    op.ldc = lda;
+    op.info = info_dev;
+    op.flags = prec_flags(stream_state(stream).prec);
+    return issue(stream, op);
+}
+
+int mu_info_reset(int *info_dev, void *stream)
+{
+    if (!info_dev)
+        return fail(MU_EINVAL, "info is NULL");
+    if (mu::fill_int(info_dev, 1, INT_MAX, (hipStream_t)stream) != 0)
+        return fail(MU_EHIP, "fill kernel launch failed");
+    return MU_OK;
+}
+
+int mu_set_update_precision(void *stream, int prec)
+{
+    if (prec < MU_PREC_FP64 || prec > MU_PREC_BF16)
+        return fail(MU_EINVAL, "unknown precision");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_streams[stream].prec = prec;
+    return MU_OK;
+}
+
+// ---- DAG ----------------------------------------------------------------------
+int mu_dag_create(mu_dag_t *dag)
+{
+    if (!dag)
+        return fail(MU_EINVAL, "dag is NULL");
+    *dag = new mu_dag_s();
+    return MU_OK;
+}
+
+int mu_dag_destroy(mu_dag_t dag)
+{
+    if (!dag)
+        return MU_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (auto &kv : g_streams)
+            if (kv.second.dag == dag)
+                kv.second.dag = nullptr;
+    }
+    delete dag;
+    return MU_OK;
+}
+
+int mu_dag_capture_begin(mu_dag_t dag, void *stream)
+{
+    if (!dag)
+        return fail(MU_EINVAL, "dag is NULL");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_streams[stream].dag = dag;
+    return MU_OK;
+}
+
+int mu_dag_capture_end(mu_dag_t dag, void *stream)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_streams.find(stream);
+    if (it == g_streams.end() || it->second.dag != dag)
+        return fail(MU_ESTATE, "stream is not being recorded by this DAG");
+    it->second.dag = nullptr;
+    return MU_OK;
+}
+
+int mu_dag_begin_task(mu_dag_t dag, int w_row, int w_col, const int *read_tiles, int nread,
+                      int *task)
+{
+    if (!dag || nread < 0 || (nread > 0 && !read_tiles))
+        return fail(MU_EINVAL, "mu_dag_begin_task: bad arguments");
+    std::vector<mu::Tile> r;
+    for (int i = 0; i < nread; i++)
+        r.push_back({read_tiles[2 * i], read_tiles[2 * i + 1]});
+    int t = dag->dag.begin_task({w_row, w_col}, r);
+    if (t < 0)
+        return fail(MU_ESTATE, dag->dag.err);
+    if (task)
+        *task = t;
+    return MU_OK;
+}
+
+int mu_dag_end_task(mu_dag_t dag)
+{
+    if (!dag)
+        return fail(MU_EINVAL, "dag is NULL");
+    if (dag->dag.end_task() != 0)
+        return fail(MU_ESTATE, dag->dag.err);
+    return MU_OK;
+}
+
+int mu_dag_num_tasks(mu_dag_t dag, int *ntasks)
+{
+    if (!dag || !ntasks)
+        return fail(MU_EINVAL, "bad arguments");
+    *ntasks = (int)dag->dag.tasks.size();
+    return MU_OK;
+}
+
+int mu_dag_task_deps(mu_dag_t dag, int task, int *deps, int max, int *ndeps)
+{
+    if (!dag || task < 0 || task >= (int)dag->dag.tasks.size() || !ndeps)
+        return fail(MU_EINVAL, "bad arguments");
+    const auto &d = dag->dag.tasks[task].deps;
+    *ndeps = (int)d.size();
+    for (int i = 0; i < (int)d.size() && i < max && deps; i++)
+        deps[i] = d[i];
+    return MU_OK;
+}
+
+int mu_dag_add_edge(mu_dag_t dag, int src, int dst)
+{
+    if (!dag || src < 0 || dst <= src || dst >= (int)dag->dag.tasks.size())
+        return fail(MU_EINVAL, "mu_dag_add_edge: edge must go from an earlier to a later task");
+    dag->dag.add_edge(src, dst);
+    return MU_OK;
+}
+
+int mu_dag_set_tag(mu_dag_t dag, int tag)
+{
+    if (!dag)
+        return fail(MU_EINVAL, "dag is NULL");
+    dag->dag.set_tag(tag);
+    return MU_OK;
+}
+
+// ---- executables --------------------------------------------------------------
+int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t *exec)
+{
+    if (!dag || !exec || mode < MU_MODE_TILED || mode > MU_MODE_STREAM)
+        return fail(MU_EINVAL, "mu_exec_create: bad arguments");
+    if (dag->dag.current() >= 0)
+        return fail(MU_ESTATE, "mu_exec_create: a task is still being recorded");
+    mu::LowerOpts lo;
+    mu::ExecOpts eo;
+    if (opts) {
+        if (opts->small_max > 0)
+            lo.small_max = std::min(opts->small_max, mu::SMALL_MAX);
+        if (opts->trsm_max > 0)
+            lo.trsm_max = std::min(opts->trsm_max, mu::TRSM_NMAX);
+        if (opts->timeout_s > 0)
+            eo.timeout_s = opts->timeout_s;
+        eo.grid = opts->grid;
+        eo.trace = opts->trace;
+    }
+    for (const auto &t : dag->dag.tasks)
+        for (const auto &op : t.ops)
+            if (op.flags & (mu::F_LOW_F32 | mu::F_LOW_BF16))
+                if (op.op != mu::OP_GEMM && op.op != mu::OP_SYRK && op.op != mu::OP_GETRF &&
+                    op.op != mu::OP_POTRF)
+                    return fail(MU_ENOTSUP, "reduced precision applies to GEMM/SYRK only");
+    mu::Exec *e = nullptr;
+    std::string err;
+    if (mu::exec_create(dag->dag, mode, lo, eo, &e, err) != 0)
+        return fail(MU_EHIP, err);
+    *exec = new mu_exec_s();
+    (*exec)->e = e;
+    return MU_OK;
+}
+
+int mu_exec_launch(mu_exec_t exec, void *stream)
+{
+    if (!exec)
+        return fail(MU_EINVAL, "exec is NULL");
+    std::string err;
+    if (mu::exec_launch(exec->e, (hipStream_t)stream, err) != 0)
+        return fail(MU_EHIP, err);
+    return MU_OK;
+}
+
+int mu_exec_check(mu_exec_t exec)
+{
+    if (!exec)
+        return fail(MU_EINVAL, "exec is NULL");
+    std::string err;
+    if (mu::exec_check(exec->e, err) != 0)
+        return fail(MU_ERUN, err);
+    return MU_OK;
+}
+
+int mu_exec_stats(mu_exec_t exec, mu_exec_stats_t *s)
+{
+    if (!exec || !s)
+        return fail(MU_EINVAL, "bad arguments");
+    const mu::Exec *e = exec->e;
+    std::memset(s, 0, sizeof *s);
+    s->mode = e->mode;
+    s->grid = e->grid;
+    s->units = (long long)e->low.units.size();
+    s->items = (long long)e->low.nitems;
+    for (const auto &v : e->low.succ)
+        s->edges += (long long)v.size();
+    s->flops = e->low.flops;
+    s->critical_path_us = e->low.critical_path;
+    for (const auto &u : e->low.units)
+        s->priority_units[u.prio & 3]++;
+    s->wall_clock_khz = (double)e->wall_khz;
+    return MU_OK;
+}
+
+int mu_exec_trace(mu_exec_t exec, unsigned long long *stamps, size_t count)
+{
+    if (!exec || !stamps)
+        return fail(MU_EINVAL, "bad arguments");
+    std::vector<uint64_t> v;
+    std::string err;
+    if (mu::exec_trace(exec->e, v, err) != 0)
+        return fail(MU_ESTATE, err);
+    if (count < v.size())
+        return fail(MU_EINVAL, "stamps buffer too small");
+    for (size_t i = 0; i < v.size(); i++)
+        stamps[i] = v[i];
+    return MU_OK;
+}
+
+int mu_exec_item_units(mu_exec_t exec, int *unit_of_item, int *task_of_unit, int *op_of_unit)
+{
+    if (!exec)
+        return fail(MU_EINVAL, "exec is NULL");
+    const mu::Lowered &L = exec->e->low;
+    for (size_t u = 0; u < L.units.size(); u++) {
+        if (task_of_unit)
+            task_of_unit[u] = L.units[u].task;
+        if (op_of_unit)
+            op_of_unit[u] = L.units[u].op;
+        if (unit_of_item)
+            for (int i = 0; i < L.units[u].nitems; i++)
+                unit_of_item[L.units[u].item_base + i] = (int)u;
+    }
+    return MU_OK;
+}
+
+int mu_exec_destroy(mu_exec_t exec)
+{
+    if (!exec)
+        return MU_OK;
+    mu::exec_destroy(exec->e);
+    delete exec;
+    return MU_OK;
+}
+
+// ---- tiled factorizations -----------------------------------------------------
+int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info_dev, int prec)
+{
+    if (!dag || !A || n <= 0 || t <= 0 || n % t != 0 || lda < n)
+        return fail(MU_EINVAL, "mu_record_tiled_lu: need n > 0, t > 0, n % t == 0, lda >= n");
+    const int b = n / t;
+    void *key = dag;  // record through the same path as a captured stream
+    int rc;
+    if ((rc = mu_dag_capture_begin(dag, key)) != 0)
+        return rc;
+    auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
+    auto rd = [](std::initializer_list<int> v) { return std::vector<int>(v); };
+    // lu_mustard.cu:343-535
+    for (int k = 0; k < t && rc == 0; k++) {
+        auto r0 = rd({k, k});
+        rc = mu_dag_begin_task(dag, k, k, r0.data(), 1, nullptr);
+        if (!rc) {
+            mu_set_update_precision(key, MU_PREC_FP64);
+            rc = mu_dgetrf_nopiv(key, b, blk(k, k), lda, info_dev);
+            if (!rc && !dag->dag.tasks.back().ops.empty())
+                dag->dag.tasks.back().ops.back().info_base = k * b;
+        }
+        if (!rc)
+            rc = mu_dag_end_task(dag);
+        for (int i = k + 1; i < t && rc == 0; i++) {
+            auto r = rd({k, k, k, i});
+            rc = mu_dag_begin_task(dag, k, i, r.data(), 2, nullptr);
+            if (!rc)
+                rc = mu_dtrsm(key, 'L', 'L', 'N', 'U', b, b, 1.0, blk(k, k), lda, blk(k, i), lda);
+            if (!rc)
+                rc = mu_dag_end_task(dag);
+        }
+        for (int i = k + 1; i < t && rc == 0; i++) {
+            auto r = rd({k, k, i, k});
+            rc = mu_dag_begin_task(dag, i, k, r.data(), 2, nullptr);
+            if (!rc)
+                rc = mu_dtrsm(key, 'R', 'U', 'N', 'N', b, b, 1.0, blk(k, k), lda, blk(i, k), lda);
+            if (!rc)
+                rc = mu_dag_end_task(dag);
+            for (int j = k + 1; j < t && rc == 0; j++) {
+                auto r3 = rd({i, k, k, j, i, j});
+                rc = mu_dag_begin_task(dag, i, j, r3.data(), 3, nullptr);
+                if (!rc) {
+                    mu_set_update_precision(key, prec);
+                    rc = mu_dgemm(key, 'N', 'N', b, b, b, -1.0, blk(i, k), lda, blk(k, j), lda,
+                                  1.0, blk(i, j), lda);
+                    mu_set_update_precision(key, MU_PREC_FP64);
+                }
+                if (!rc)
+                    rc = mu_dag_end_task(dag);
+            }
+        }
+    }
+    int rc2 = mu_dag_capture_end(dag, key);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_streams.erase(key);
+    }
+    return rc ? rc : rc2;
+}
+
+int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int *info_dev,
+                             int prec)
+{
+    if (!dag || !A || n <= 0 || t <= 0 || n % t != 0 || lda < n)
+        return fail(MU_EINVAL,
+                    "mu_record_tiled_cholesky: need n > 0, t > 0, n % t == 0, lda >= n");
+    const int b = n / t;
+    void *key = dag;
+    int rc;
+    if ((rc = mu_dag_capture_begin(dag, key)) != 0)
+        return rc;
+    auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
+    auto rd = [](std::initializer_list<int> v) { return std::vector<int>(v); };
+    // cholesky_mustard.cu:221-400
+    for (int k = 0; k < t && rc == 0; k++) {
+        auto r0 = rd({k, k});
+        rc = mu_dag_begin_task(dag, k, k, r0.data(), 1, nullptr);
+        if (!rc) {
+            rc = mu_dpotrf(key, 'L', b, blk(k, k), lda, info_dev);
+            if (!rc && !dag->dag.tasks.back().ops.empty())
+                dag->dag.tasks.back().ops.back().info_base = k * b;
+        }
+        if (!rc)
+            rc = mu_dag_end_task(dag);
+        for (int i = k + 1; i < t && rc == 0; i++) {
+            auto r = rd({k, k, i, k});
+            rc = mu_dag_begin_task(dag, i, k, r.data(), 2, nullptr);
+            if (!rc)
+                rc = mu_dtrsm(key, 'R', 'L', 'T', 'N', b, b, 1.0, blk(k, k), lda, blk(i, k), lda);
+            if (!rc)
+                rc = mu_dag_end_task(dag);
+        }
+        for (int i = k + 1; i < t && rc == 0; i++) {
+            auto r = rd({i, i, i, k});
+            rc = mu_dag_begin_task(dag, i, i, r.data(), 2, nullptr);
+            if (!rc) {
+                mu_set_update_precision(key, prec);
+                rc = mu_dsyrk(key, 'L', 'N', b, b, -1.0, blk(i, k), lda, 1.0, blk(i, i), lda);
+                mu_set_update_precision(key, MU_PREC_FP64);
+            }
+            if (!rc)
+                rc = mu_dag_end_task(dag);
+            for (int j = i + 1; j < t && rc == 0; j++) {
+                auto r3 = rd({j, i, j, k, i, k});
+                rc = mu_dag_begin_task(dag, j, i, r3.data(), 3, nullptr);
+                if (!rc) {
+                    mu_set_update_precision(key, prec);
+                    rc = mu_dgemm(key, 'N', 'T', b, b, b, -1.0, blk(j, k), lda, blk(i, k), lda,
+                                  1.0, blk(j, i), lda);
+                    mu_set_update_precision(key, MU_PREC_FP64);
+                }
+                if (!rc)
+                    rc = mu_dag_end_task(dag);
+            }
+        }
+    }
+    int rc2 = mu_dag_capture_end(dag, key);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_streams.erase(key);
+    }
+    return rc ? rc : rc2;
+}
+
+// ---- generators -------------------------------------------------------------
+void mu_gen_ref_host(double *h_A, size_t n)
+{
+    // include/gen.h:23-38: same seed, same loop order, same int indices
+    srand(420);
+    for (int i = 0; i < (int)n; i++)
+        for (int j = i; j < (int)n; j++)
+            h_A[(size_t)i * n + j] = (double)rand() / (double)RAND_MAX;
+    for (int i = 0; i < (int)n; i++)
+        for (int j = i; j >= 0; j--)
+            h_A[(size_t)i * n + j] = h_A[(size_t)j * n + i];
+    for (int i = 0; i < (int)n; i++)
+        h_A[(size_t)i * n + i] = h_A[(size_t)i * n + i] + (double)n;
+}
+
+int mu_gen_uniform(double *M, size_t count, uint64_t seed, double lo, double hi, void *stream)
+{
+    if (!M)
+        return fail(MU_EINVAL, "M is NULL");
+    if (mu::gen_uniform_device(M, count, seed, lo, hi, (hipStream_t)stream) != 0)
+        return fail(MU_EHIP, "generator launch failed");
+    return MU_OK;
+}
+
+int mu_gen_spd(double *A, int n, int lda, uint64_t seed, double *work, void *stream)
+{
+    if (!A || !work || n <= 0 || lda < n)
+        return fail(MU_EINVAL, "mu_gen_spd: bad arguments");
+    int rc = mu_gen_uniform(work, (size_t)n * n, seed, -1.0, 1.0, stream);
+    if (rc)
+        return rc;
+    rc = mu_dgemm(stream, 'N', 'T', n, n, n, 1.0, work, n, work, n, 0.0, A, lda);
+    if (rc)
+        return rc;
+    if (mu::add_diag_scale(A, n, lda, (double)n, (double)n, (hipStream_t)stream) != 0 ||
+        mu::symmetrize_lower(A, n, lda, (hipStream_t)stream) != 0)
+        return fail(MU_EHIP, "kernel launch failed");
+    return MU_OK;
+}
+
+// ---- verification -----------------------------------------------------------
+double mu_verify_lu_l1_host(const double *A, const double *F, int n)
+{
+    // include/verify.h:17-71: row-major unit-lower L and upper U, L*U by
+    // triple loop, sum |A - LU|.
+    const size_t nn = (size_t)n * n;
+    std::vector<double> L(F, F + nn), U(nn, 0.0), P(nn, 0.0);
+    for (int i = 0; i < n; i++) {
+        for (int j = i; j < n; j++) {
+            std::swap(L[i + (size_t)j * n], L[(size_t)i * n + j]);
+            U[(size_t)i * n + j] = L[(size_t)i * n + j];
+            L[(size_t)i * n + j] = 0;
+        }
+        L[(size_t)i * n + i] = 1;
+    }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++)
+                P[(size_t)i * n + j] += L[(size_t)i * n + k] * U[(size_t)k * n + j];
+    double e = 0;
+    for (size_t x = 0; x < nn; x++)
+        e += std::fabs(A[x] - P[x]);
+    return e;
+}
+
+double mu_verify_chol_l1_host(const double *A, const double *F, int n)
+{
+    // include/verify.h:79-129
+    const size_t nn = (size_t)n * n;
+    std::vector<double> L(F, F + nn), P(nn, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) {
+            L[i + (size_t)j * n] = 0;
+            std::swap(L[i + (size_t)j * n], L[(size_t)i * n + j]);
+        }
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            for (int k = 0; k < n; k++)
+                P[(size_t)i * n + j] += L[(size_t)i * n + k] * L[k + (size_t)j * n];
+    double e = 0;
+    for (size_t x = 0; x < nn; x++)
+        e += std::fabs(A[x] - P[x]);
+    return e;
+}
+
+int mu_scaled_residual(const double *A, const double *F, int n, int lda, int chol, double *work,
+                       double *result, void *stream)
+{
+    if (!A || !F || !work || !result || n <= 0 || lda < n)
+        return fail(MU_EINVAL, "mu_scaled_residual: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nn = (size_t)n * n;
+    double *L = work, *U = work + nn, *R = work + 2 * nn;
+    double *acc = nullptr;
+    HIPTRY(hipMalloc(&acc, 2 * sizeof(double)));
+    HIPTRY(hipMemsetAsync(acc, 0, 2 * sizeof(double), s));
+    int rc = 0;
+    if (mu::split_lu(F, n, lda, L, U, n, chol, s) || mu::copy_matrix(A, lda, R, n, n, n, s))
+        rc = fail(MU_EHIP, "kernel launch failed");
+    if (!rc)
+        rc = chol ? mu_dgemm(stream, 'N', 'T', n, n, n, -1.0, L, n, L, n, 1.0, R, n)
+                  : mu_dgemm(stream, 'N', 'N', n, n, n, -1.0, L, n, U, n, 1.0, R, n);
+    if (!rc && (mu::sub_frob2(R, n, nullptr, 0, n, n, acc, s) ||
+                mu::sub_frob2(A, lda, nullptr, 0, n, n, acc + 1, s)))
+        rc = fail(MU_EHIP, "kernel launch failed");
+    double h[2] = {0, 0};
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(s);
+        if (e != hipSuccess)
+            rc = fail(MU_EHIP, hipGetErrorString(e));
+    }
+    (void)hipFree(acc);
+    if (!rc)
+        *result = std::sqrt(h[0]) / (std::sqrt(h[1]) * n * DBL_EPSILON);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// mu_dag.hpp -- host-side task recording and lowering to device units.
+//
+// Recording follows the reference's TiledGraphCreator (mustard/mustard.h:
+// 132-384): a task names the tile it writes and the tiles it reads; its
+// dependencies are the sorted, de-duplicated last writers of those tiles
+// (getSubgraphDependencies, mustard/mustard.h:279-296).  Between begin and end
+// of a task the caller issues tile operations (our replacements of the cuBLAS
+// / cuSOLVER calls); they are recorded as OpRecs instead of being captured
+// from a stream.
+#pragma once
+#include <cstdint>
+#include <map>
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "mu_types.h"
+
+namespace mu {
+
+// One tile operation at task level (before lowering).
+struct OpRec {
+    int32_t op = OP_NONE;
+    int32_t flags = 0;
+    int32_t m = 0, n = 0, k = 0;  // GEMM m,n,k; SYRK n (in m and n), k; TRSM: m = order, n = nrhs
+    int32_t lda = 0, ldb = 0, ldc = 0;
+    double alpha = -1.0, beta = 1.0;
+    const double *a = nullptr;
+    const double *b = nullptr;
+    double *c = nullptr;
+    int32_t *info = nullptr;
+    int32_t info_base = 0;
+};
+
+typedef std::pair<int, int> Tile;
+
+struct TaskRec {
+    Tile w;
+    std::vector<Tile> r;
+    std::vector<int> deps;  // sorted unique task ids
+    std::vector<OpRec> ops;
+    int tag = 0;            // problem id (batched runs)
+};
+
+class Dag {
+  public:
+    // beginCaptureOperation / endCaptureOperation (mustard/mustard.h:148-181).
+    int begin_task(Tile w, const std::vector<Tile> &r);
+    int end_task();
+    int add_op(const OpRec &op);
+    void add_edge(int src, int dst);  // insertDependencyKernel (mustard.h:197-211)
+    int current() const { return cur_; }
+    void set_tag(int tag) { tag_ = tag; }
+
+    std::vector<TaskRec> tasks;
+    std::string err;
+
+  private:
+    std::map<std::tuple<int, int, int>, int> last_writer_;  // (tag, row, col) -> task
+    int cur_ = -1;
+    int tag_ = 0;
+};
+
+struct LowerOpts {
+    int small_max = SMALL_MAX;  // diagonal factorizations above this nest into sub-tiles
+    int trsm_max = TRSM_NMAX;   // triangular solves above this nest into blocks
+    int nprio = NPRIO;
+};
+
+struct Lowered {
+    std::vector<UnitDesc> units;
+    std::vector<std::vector<uint32_t>> succ;  // per unit
+    std::vector<int> task_first_unit;         // per task: first unit id
+    std::vector<int> task_nunits;
+    uint64_t nitems = 0;
+    double flops = 0.0;  // algorithmic flop count of the recorded ops
+    double critical_path = 0.0;
+};
+
+// Lower the recorded DAG to device units (nested sub-DAGs for large
+// factorizations and solves), assign priorities and number the items.
+bool lower(const Dag &dag, const LowerOpts &opts, Lowered &out, std::string &err);
+
+// Number of items of a unit as the device decodes it.
+int unit_items(const UnitDesc &u);
+
+// Algorithmic flops of one recorded operation.
+double op_flops(const OpRec &op);
+
+}  // namespace mu

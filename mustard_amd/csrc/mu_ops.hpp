@@ -1,0 +1,496 @@
+// mu_ops.hpp -- gfx950 work-item bodies of the tile operations.
+//
+// One work item runs on one workgroup of 256 threads (4 wave64s).  The same
+// bodies serve the per-unit grid kernels of the hipGraph mode and the
+// persistent scheduler kernel.
+//
+// Arithmetic replaced (all column-major, leading dimension per operand):
+//   GEMM  cublasGemmEx 64F  (mustard/lu_mustard.cu:511-522, cholesky_mustard.cu:376-387)
+//   SYRK  cublasDsyrk LOWER (mustard/cholesky_mustard.cu:327-333)
+//   TRSM  cublasDtrsm       (mustard/lu_mustard.cu:413-422, 459-468; cholesky_mustard.cu:281-290)
+//   GETRF cusolverDnDgetrf, devIpiv = NULL (mustard/lu_mustard.cu:360-369)
+//   POTRF cusolverDnDpotrf LOWER (mustard/cholesky_mustard.cu:238-246)
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "mu_types.h"
+
+namespace mu {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4_f64.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// it receives D[4*e + l/16][l%16] in element e (register e holds rows 4e..4e+3).
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM / SYRK output block: C[i0:i0+128, j0:j0+128] (+)= alpha op(A) op(B)
+// ---------------------------------------------------------------------------
+// LDS layouts.  An operand whose stored columns run along the output
+// dimension ("i-contiguous": A, or B^T) is kept as S[k][i] with a pitch of
+// PI doubles; one whose stored columns run along k ("k-contiguous": A^T, or B)
+// is kept as S[i][k] with a pitch of PK doubles.  Both pitches make the
+// fragment reads of a half-wave fall in distinct banks.
+constexpr int PI = GEMM_BM + 16;
+constexpr int PK = GEMM_BK + 1;
+constexpr int LDS_I = GEMM_BK * PI;   // doubles per i-contiguous buffer
+constexpr int LDS_K = GEMM_BM * PK;   // doubles per k-contiguous buffer
+
+// Global -> registers, i-contiguous operand: element (r, k) at p[r + k*ld],
+// r < rows, k < kcols.  Thread t covers rows 2(t%64), 2(t%64)+1 of columns
+// t/64 + 4q, q = 0..3.
+template <bool VEC>
+__device__ __forceinline__ void ld_icontig(double (&v)[8], const double *p, int ld, int rows,
+                                           int kcols)
+{
+    const int t = threadIdx.x, r = (t & 63) * 2, c0 = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = c0 + 4 * q;
+        const double *s = p + r + (size_t)c * ld;
+        if (VEC && c < kcols && r + 1 < rows) {
+            double2 x = *reinterpret_cast<const double2 *>(s);
+            v[2 * q] = x.x;
+            v[2 * q + 1] = x.y;
+        } else {
+            v[2 * q] = (c < kcols && r < rows) ? s[0] : 0.0;
+            v[2 * q + 1] = (c < kcols && r + 1 < rows) ? s[1] : 0.0;
+        }
+    }
+}
+
+__device__ __forceinline__ void st_icontig(double *S, const double (&v)[8])
+{
+    const int t = threadIdx.x, r = (t & 63) * 2, c0 = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double2 x;
+        x.x = v[2 * q];
+        x.y = v[2 * q + 1];
+        *reinterpret_cast<double2 *>(S + (c0 + 4 * q) * PI + r) = x;
+    }
+}
+
+// Global -> registers, k-contiguous operand: element (r, k) at p[k + r*ld].
+// Thread t covers k = 2(t%8), 2(t%8)+1 of rows t/8 + 32q, q = 0..3.
+template <bool VEC>
+__device__ __forceinline__ void ld_kcontig(double (&v)[8], const double *p, int ld, int rows,
+                                           int kcols)
+{
+    const int t = threadIdx.x, k = (t & 7) * 2, r0 = t >> 3;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int r = r0 + 32 * q;
+        const double *s = p + k + (size_t)r * ld;
+        if (VEC && r < rows && k + 1 < kcols) {
+            double2 x = *reinterpret_cast<const double2 *>(s);
+            v[2 * q] = x.x;
+            v[2 * q + 1] = x.y;
+        } else {
+            v[2 * q] = (r < rows && k < kcols) ? s[0] : 0.0;
+            v[2 * q + 1] = (r < rows && k + 1 < kcols) ? s[1] : 0.0;
+        }
+    }
+}
+
+__device__ __forceinline__ void st_kcontig(double *S, const double (&v)[8])
+{
+    const int t = threadIdx.x, k = (t & 7) * 2, r0 = t >> 3;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        double *d = S + (r0 + 32 * q) * PK + k;
+        d[0] = v[2 * q];
+        d[1] = v[2 * q + 1];
+    }
+}
+
+// op(A)(i, k) read from the LDS copy.
+template <bool TA>
+__device__ __forceinline__ double frag_a(const double *S, int i, int k)
+{
+    return TA ? S[i * PK + k] : S[k * PI + i];
+}
+// op(B)(k, j) read from the LDS copy.
+template <bool TB>
+__device__ __forceinline__ double frag_b(const double *S, int k, int j)
+{
+    return TB ? S[k * PI + j] : S[j * PK + k];
+}
+
+template <bool TA, bool TB, bool VEC>
+__device__ void gemm_block(const UnitDesc &u, int bi, int bj, bool lower_only, double *smem)
+{
+    constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = GEMM_BK;
+    constexpr int SA = TA ? LDS_K : LDS_I;
+    constexpr int SB = TB ? LDS_I : LDS_K;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;  // 2 x 2 waves of 64 x 64
+    const int i0 = bi * BM, j0 = bj * BN;
+    const int rows = min(BM, u.m - i0), cols = min(BN, u.n - j0);
+    const int K = u.k;
+    // op(A)(i, k): TA ? a[k + i*lda] : a[i + k*lda];  op(B)(k, j): TB ? b[j + k*ldb] : b[k + j*ldb]
+    const double *A = TA ? u.a + (size_t)i0 * u.lda : u.a + i0;
+    const double *B = TB ? u.b + j0 : u.b + (size_t)j0 * u.ldb;
+    double *As0 = smem, *As1 = smem + SA, *Bs0 = smem + 2 * SA, *Bs1 = smem + 2 * SA + SB;
+
+    v4d acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+            acc[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    double ra[8], rb[8];
+    auto load = [&](int k0) {
+        const int kc = K - k0;
+        if (TA)
+            ld_kcontig<VEC>(ra, A + k0, u.lda, rows, kc);
+        else
+            ld_icontig<VEC>(ra, A + (size_t)k0 * u.lda, u.lda, rows, kc);
+        if (TB)
+            ld_icontig<VEC>(rb, B + (size_t)k0 * u.ldb, u.ldb, cols, kc);
+        else
+            ld_kcontig<VEC>(rb, B + k0, u.ldb, cols, kc);
+    };
+    auto store = [&](double *As, double *Bs) {
+        if (TA)
+            st_kcontig(As, ra);
+        else
+            st_icontig(As, ra);
+        if (TB)
+            st_icontig(Bs, rb);
+        else
+            st_kcontig(Bs, rb);
+    };
+
+    const int nk = (K + BK - 1) / BK;
+    load(0);
+    store(As0, Bs0);
+    __syncthreads();
+    const int li = lane & 15, lk = lane >> 4;
+    for (int kb = 0; kb < nk; kb++) {
+        const double *As = (kb & 1) ? As1 : As0;
+        const double *Bs = (kb & 1) ? Bs1 : Bs0;
+        if (kb + 1 < nk)
+            load((kb + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double fa[4], fb[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+                fa[x] = frag_a<TA>(As, wm * 64 + x * 16 + li, kk + lk);
+#pragma unroll
+            for (int y = 0; y < 4; y++)
+                fb[y] = frag_b<TB>(Bs, kk + lk, wn * 64 + y * 16 + li);
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++)
+                    acc[x][y] = mfma_f64(fa[x], fb[y], acc[x][y]);
+        }
+        if (kb + 1 < nk) {
+            if (kb & 1)
+                store(As0, Bs0);
+            else
+                store(As1, Bs1);
+        }
+        __syncthreads();
+    }
+
+    const double alpha = u.alpha, beta = u.beta;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = wm * 64 + x * 16 + 4 * e + lk;
+                const int c = wn * 64 + y * 16 + li;
+                if (r < rows && c < cols && (!lower_only || i0 + r >= j0 + c)) {
+                    double *p = u.c + (i0 + r) + (size_t)(j0 + c) * u.ldc;
+                    const double old = beta == 0.0 ? 0.0 : beta * *p;
+                    *p = old + alpha * acc[x][y][e];
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Panel triangular solve.  Solves T Y = Z in place for one panel of up to 16
+// right-hand sides, T lower triangular of order n <= TRSM_NMAX:
+//   left-lower-unit   X = L^-1 B  : T = L,   Z = B (columns)
+//   right-upper       X = B U^-1  : T = U^T, Z = B^T (rows of B)
+//   right-lower-trans X = B L^-T  : T = L,   Z = B^T (rows of B)
+// With REV the row order of T and Z is reversed, which turns an upper
+// triangular solve (back substitution) into the same lower-triangular form.
+// Z is scaled by alpha on load.  The panel lives in LDS; each 16-row step solves its diagonal block with one
+// thread per right-hand side and updates the rows below with FP64 MFMA.
+// ---------------------------------------------------------------------------
+template <bool TT, bool ZR, bool UNIT, bool REV>
+__device__ void trsm_panel(const UnitDesc &u, int panel, double *smem)
+{
+    const int n = u.m, p0 = panel * TRSM_W, w = min(TRSM_W, u.n - p0);
+    const int np = (n + 15) & ~15;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *Y = smem;                   // [np][16]
+    double *Td = smem + TRSM_NMAX * 16; // [16][17]
+    const double *t = u.a;
+    const size_t ldt = u.lda, ldz = u.ldc;
+    double *z = u.c;
+    const double alpha = u.alpha;
+    auto tv = [&](int r, int c) -> double {
+        if (REV) {
+            r = n - 1 - r;
+            c = n - 1 - c;
+        }
+        return TT ? t[c + r * ldt] : t[r + c * ldt];
+    };
+    auto zp = [&](int r, int c) -> double * {
+        if (REV)
+            r = n - 1 - r;
+        return ZR ? z + (p0 + c) + r * ldz : z + r + (p0 + c) * ldz;
+    };
+
+    for (int idx = tid; idx < np * 16; idx += WG_THREADS) {
+        int r, c;
+        if (ZR) {
+            c = idx & 15;
+            r = idx >> 4;
+        } else {
+            r = idx % np;
+            c = idx / np;
+        }
+        Y[r * 16 + c] = (r < n && c < w) ? alpha * *zp(r, c) : 0.0;
+    }
+    __syncthreads();
+
+    const int li = lane & 15, lk = lane >> 4;
+    for (int jb = 0; jb < n; jb += 16) {
+        {
+            const int r = tid >> 4, c = tid & 15;
+            Td[r * 17 + c] = (jb + r < n && c <= r) ? tv(jb + r, jb + c) : 0.0;
+        }
+        __syncthreads();
+        if (tid < 16) {
+            const int nb = min(16, n - jb);
+            double y[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                y[r] = Y[(jb + r) * 16 + tid];
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                if (r < nb) {
+                    double s = y[r];
+#pragma unroll
+                    for (int q = 0; q < r; q++)
+                        s -= Td[r * 17 + q] * y[q];
+                    if (!UNIT)
+                        s /= Td[r * 17 + r];
+                    y[r] = s;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                if (r < nb)
+                    Y[(jb + r) * 16 + tid] = y[r];
+        }
+        __syncthreads();
+        // Rows below the diagonal block: Y[rb] -= T[rb, jb] Y[jb].  All the
+        // triangle fragments of this wave are loaded before the MFMAs run.
+        constexpr int MAXB = TRSM_NMAX / 64;
+        double fa[MAXB][4];
+#pragma unroll
+        for (int i = 0; i < MAXB; i++) {
+            const int rb = jb + 16 + 16 * wave + 64 * i;
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int r = rb + li, c = jb + 4 * kk + lk;
+                fa[i][kk] = (rb < np && r < n && c < n) ? tv(r, c) : 0.0;
+            }
+        }
+        double fb[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+            fb[kk] = Y[(jb + 4 * kk + lk) * 16 + li];
+#pragma unroll
+        for (int i = 0; i < MAXB; i++) {
+            const int rb = jb + 16 + 16 * wave + 64 * i;
+            if (rb < np) {
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++)
+                    acc = mfma_f64(fa[i][kk], fb[kk], acc);
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    Y[(rb + 4 * e + lk) * 16 + li] -= acc[e];
+            }
+        }
+        __syncthreads();
+    }
+
+    for (int idx = tid; idx < np * 16; idx += WG_THREADS) {
+        int r, c;
+        if (ZR) {
+            c = idx & 15;
+            r = idx >> 4;
+        } else {
+            r = idx % np;
+            c = idx / np;
+        }
+        if (r < n && c < w)
+            *zp(r, c) = Y[r * 16 + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Small factorizations of an n x n tile (n <= SMALL_MAX) held in LDS.
+// ---------------------------------------------------------------------------
+constexpr int SP = SMALL_MAX + 1;  // LDS pitch (column-major copy S[c*SP + r])
+
+__device__ __forceinline__ void report_info(const UnitDesc &u, int col)
+{
+    if (u.info)
+        __hip_atomic_fetch_min(u.info, u.info_base + col + 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void getrf_small(const UnitDesc &u, double *smem)
+{
+    const int n = u.m, tid = threadIdx.x;
+    double *S = smem;
+    double *a = u.c;
+    const size_t lda = u.ldc;
+    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
+        const int r = idx % n, c = idx / n;
+        S[c * SP + r] = a[r + c * lda];
+    }
+    __syncthreads();
+    for (int c = 0; c < n; c++) {
+        const double piv = S[c * SP + c];
+        if (tid == 0 && piv == 0.0)
+            report_info(u, c);
+        for (int r = c + 1 + tid; r < n; r += WG_THREADS)
+            S[c * SP + r] /= piv;
+        __syncthreads();
+        const int m = n - c - 1;
+        for (int e = tid; e < m * m; e += WG_THREADS) {
+            const int r = c + 1 + e % m, cc = c + 1 + e / m;
+            S[cc * SP + r] -= S[c * SP + r] * S[cc * SP + c];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
+        const int r = idx % n, c = idx / n;
+        a[r + c * lda] = S[c * SP + r];
+    }
+}
+
+__device__ void potrf_small(const UnitDesc &u, double *smem)
+{
+    const int n = u.m, tid = threadIdx.x;
+    double *S = smem;
+    double *a = u.c;
+    const size_t lda = u.ldc;
+    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
+        const int r = idx % n, c = idx / n;
+        if (r >= c)
+            S[c * SP + r] = a[r + c * lda];
+    }
+    __syncthreads();
+    for (int c = 0; c < n; c++) {
+        double d = S[c * SP + c];
+        if (tid == 0 && !(d > 0.0))
+            report_info(u, c);
+        d = sqrt(d);
+        __syncthreads();
+        for (int r = c + tid; r < n; r += WG_THREADS)
+            S[c * SP + r] = (r == c) ? d : S[c * SP + r] / d;
+        __syncthreads();
+        const int m = n - c - 1;
+        for (int e = tid; e < m * m; e += WG_THREADS) {
+            const int r = c + 1 + e % m, cc = c + 1 + e / m;
+            if (r >= cc)
+                S[cc * SP + r] -= S[c * SP + r] * S[c * SP + cc];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
+        const int r = idx % n, c = idx / n;
+        if (r >= c)
+            a[r + c * lda] = S[c * SP + r];
+    }
+}
+
+// LDS (doubles) one workgroup needs for any item.
+constexpr int LDS_GEMM = 2 * (LDS_I > LDS_K ? LDS_I : LDS_K) * 2;
+constexpr int LDS_TRSM = TRSM_NMAX * 16 + 16 * 17;
+constexpr int LDS_SMALL = SMALL_MAX * SP;
+constexpr int LDS_DOUBLES = LDS_GEMM > LDS_TRSM ? (LDS_GEMM > LDS_SMALL ? LDS_GEMM : LDS_SMALL)
+                                                : (LDS_TRSM > LDS_SMALL ? LDS_TRSM : LDS_SMALL);
+
+// SYRK item i -> (bi, bj), bi >= bj, column-major over the lower blocks.
+__device__ __forceinline__ void syrk_block(int g, int i, int &bi, int &bj)
+{
+    int j = 0;
+    while (i >= g - j) {
+        i -= g - j;
+        j++;
+    }
+    bj = j;
+    bi = j + i;
+}
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_dispatch_vec(const UnitDesc &u, int bi, int bj, bool lo,
+                                                  double *smem)
+{
+    if (u.flags & F_VEC)
+        gemm_block<TA, TB, true>(u, bi, bj, lo, smem);
+    else
+        gemm_block<TA, TB, false>(u, bi, bj, lo, smem);
+}
+
+// Execute local item `it` of unit `u`.
+__device__ void run_item(const UnitDesc &u, int it, double *smem)
+{
+    switch (u.op) {
+    case OP_GEMM: {
+        const int bi = it % u.gm, bj = it / u.gm;
+        switch (u.flags & (F_TA | F_TB)) {
+        case 0: gemm_dispatch_vec<false, false>(u, bi, bj, false, smem); break;
+        case F_TB: gemm_dispatch_vec<false, true>(u, bi, bj, false, smem); break;
+        case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
+        default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
+        }
+        break;
+    }
+    case OP_SYRK: {
+        int bi, bj;
+        syrk_block(u.gm, it, bi, bj);
+        gemm_dispatch_vec<false, true>(u, bi, bj, bi == bj, smem);
+        break;
+    }
+    case OP_TRSM:
+        switch ((u.flags >> 3) & 0x27) {  // T_TRANS, Z_ROWS, UNIT, REV
+#define MU_TRSM_CASE(bits)                                                                  \
+    case bits:                                                                              \
+        trsm_panel<(bits & 1) != 0, (bits & 2) != 0, (bits & 4) != 0, (bits & 0x20) != 0>( \
+            u, it, smem);                                                                   \
+        break;
+            MU_TRSM_CASE(0x00) MU_TRSM_CASE(0x01) MU_TRSM_CASE(0x02) MU_TRSM_CASE(0x03)
+            MU_TRSM_CASE(0x04) MU_TRSM_CASE(0x05) MU_TRSM_CASE(0x06) MU_TRSM_CASE(0x07)
+            MU_TRSM_CASE(0x20) MU_TRSM_CASE(0x21) MU_TRSM_CASE(0x22) MU_TRSM_CASE(0x23)
+            MU_TRSM_CASE(0x24) MU_TRSM_CASE(0x25) MU_TRSM_CASE(0x26) MU_TRSM_CASE(0x27)
+#undef MU_TRSM_CASE
+        default: break;
+        }
+        break;
+    case OP_GETRF: getrf_small(u, smem); break;
+    case OP_POTRF: potrf_small(u, smem); break;
+    default: break;
+    }
+}
+
+}  // namespace mu

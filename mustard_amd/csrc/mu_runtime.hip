@@ -1,0 +1,547 @@
+// mu_runtime.hip -- kernels and host runtime for lowered unit DAGs on gfx950.
+//
+// Replaces the reference's device runtime (mustard/mustard.h:57-129,
+// mustard/broker_queue.h): instead of a one-thread scheduler kernel that
+// device-launches one CUDA sub-graph per task (HIP has no device-side graph
+// launch), every workgroup of one persistent kernel pulls ready work items
+// from HBM-resident queue rings, runs them inline, and on completion
+// decrements the dependency counters of the successor units, queueing the
+// items of every unit whose counter reaches zero.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "mu_ops.hpp"
+#include "mu_runtime.hpp"
+
+namespace mu {
+
+#define HIPCHK(x)                                                        \
+    do {                                                                 \
+        hipError_t e_ = (x);                                             \
+        if (e_ != hipSuccess) {                                          \
+            err = std::string(#x) + ": " + hipGetErrorString(e_);        \
+            return -1;                                                   \
+        }                                                                \
+    } while (0)
+
+#define AGENT __HIP_MEMORY_SCOPE_AGENT
+
+// ---------------------------------------------------------------------------
+// per-unit grid kernel (graph and stream modes): item = blockIdx.x
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(WG_THREADS) void k_unit(UnitDesc d)
+{
+    __shared__ double smem[LDS_DOUBLES];
+    run_item(d, blockIdx.x, smem);
+}
+
+// ---------------------------------------------------------------------------
+// persistent scheduler
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT);
+}
+
+// Queue all items of unit u on its priority ring.  The ring never wraps: it
+// holds every item of its priority once per run.
+__device__ void push_unit(const ExecView &ev, uint32_t u)
+{
+    const UnitDesc *d = ev.units + u;
+    const int p = d->prio;
+    const uint32_t cnt = (uint32_t)d->nitems, base = d->item_base;
+    const uint32_t pos = __hip_atomic_fetch_add(&ev.st->tail[p], cnt, __ATOMIC_RELAXED, AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    for (uint32_t r = 0; r < cnt; r++)
+        __hip_atomic_store(ev.slots[p] + pos + r, base + r + 1, __ATOMIC_RELAXED, AGENT);
+}
+
+// One attempt on ring p: an item id, -1 when the ring is empty.
+__device__ int try_pop(const ExecView &ev, int p, uint64_t deadline)
+{
+    uint32_t *head = &ev.st->head[p];
+    for (;;) {
+        uint32_t h = ld_relaxed(head);
+        const uint32_t t = ld_relaxed(&ev.st->tail[p]);
+        if (h >= t)
+            return -1;
+        if (__hip_atomic_compare_exchange_strong(head, &h, h + 1, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, AGENT)) {
+            // slot h is reserved by a producer that is writing it now
+            uint32_t v;
+            while ((v = __hip_atomic_load(ev.slots[p] + h, __ATOMIC_ACQUIRE, AGENT)) == 0) {
+                if (wall_clock64() > deadline || ld_relaxed(&ev.st->abort))
+                    return -1;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            return (int)(v - 1);
+        }
+    }
+}
+
+// Next item for this workgroup, or -1 when the run is over (or aborted).
+__device__ int pop_any(const ExecView &ev)
+{
+    uint64_t t0 = wall_clock64();
+    uint32_t seen = ld_relaxed(&ev.st->done);
+    for (;;) {
+        if (ld_relaxed(&ev.st->abort))
+            return -1;
+        for (int p = 0; p < NPRIO; p++) {
+            const int r = try_pop(ev, p, t0 + ev.timeout_ticks);
+            if (r >= 0)
+                return r;
+        }
+        const uint32_t d = ld_relaxed(&ev.st->done);
+        if (d >= ev.nitems)
+            return -1;
+        const uint64_t now = wall_clock64();
+        if (d != seen) {
+            seen = d;
+            t0 = now;
+        } else if (now - t0 > ev.timeout_ticks) {
+            __hip_atomic_store(&ev.st->err, 1u, __ATOMIC_RELAXED, AGENT);
+            __hip_atomic_store(&ev.st->abort, 1u, __ATOMIC_RELAXED, AGENT);
+            return -1;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+__global__ __launch_bounds__(WG_THREADS) void k_persistent(ExecView ev)
+{
+    __shared__ double smem[LDS_DOUBLES];
+    __shared__ int s_item, s_unit, s_last;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) {
+            const int it = pop_any(ev);
+            s_item = it;
+            s_unit = it >= 0 ? (int)ev.item_unit[it] : -1;
+        }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        if (item < 0)
+            break;
+        const UnitDesc *d = ev.units + u;
+        if (ev.trace && tid == 0)
+            ev.trace[2 * (size_t)item] = wall_clock64();
+        run_item(*d, item - (int)d->item_base, smem);
+        // make this item's tile writes visible device-wide before it is
+        // counted as complete
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0) {
+            if (ev.trace)
+                ev.trace[2 * (size_t)item + 1] = wall_clock64();
+            s_last = __hip_atomic_fetch_add(ev.items_left + u, -1, __ATOMIC_ACQ_REL, AGENT) == 1;
+        }
+        __syncthreads();
+        if (s_last) {
+            const uint32_t b = d->succ_begin, e = d->succ_end;
+            for (uint32_t s = b + tid; s < e; s += WG_THREADS) {
+                const uint32_t v = ev.succ[s];
+                if (__hip_atomic_fetch_add(ev.dep_count + v, -1, __ATOMIC_ACQ_REL, AGENT) == 1)
+                    push_unit(ev, v);
+            }
+        }
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_fetch_add(&ev.st->done, 1u, __ATOMIC_RELEASE, AGENT);
+    }
+}
+
+struct ResetArgs {
+    uint32_t init_tail[NPRIO];
+    const uint32_t *slots_init;
+    uint32_t *slots_flat;
+    uint32_t slots_total;
+};
+
+__global__ void k_reset(ExecView ev, ResetArgs ra)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (size_t u = i0; u < ev.nunits; u += stride) {
+        ev.dep_count[u] = ev.units[u].ndeps;
+        ev.items_left[u] = ev.units[u].nitems;
+    }
+    for (size_t s = i0; s < ra.slots_total; s += stride)
+        ra.slots_flat[s] = ra.slots_init[s];
+    if (i0 == 0) {
+        for (int p = 0; p < NPRIO; p++) {
+            ev.st->head[p] = 0;
+            ev.st->tail[p] = ra.init_tail[p];
+        }
+        ev.st->done = 0;
+        ev.st->abort = 0;
+        ev.st->err = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int run_units(const Lowered &low, hipStream_t s, std::string &err)
+{
+    for (const UnitDesc &d : low.units) {
+        if (d.nitems <= 0)
+            continue;
+        hipLaunchKernelGGL(k_unit, dim3(d.nitems), dim3(WG_THREADS), 0, s, d);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts &eopts,
+                Exec **out, std::string &err)
+{
+    Exec *e = new Exec();
+    e->mode = mode;
+    if (!lower(dag, lopts, e->low, err)) {
+        delete e;
+        return -1;
+    }
+    Lowered &L = e->low;
+    const uint32_t nu = (uint32_t)L.units.size();
+    HIPCHK(hipGetDevice(&e->device));
+
+    // CSR successors and per-priority ring capacities
+    std::vector<uint32_t> succ;
+    uint32_t cap[NPRIO] = {0, 0, 0, 0};
+    for (uint32_t u = 0; u < nu; u++) {
+        L.units[u].succ_begin = (uint32_t)succ.size();
+        succ.insert(succ.end(), L.succ[u].begin(), L.succ[u].end());
+        L.units[u].succ_end = (uint32_t)succ.size();
+        cap[L.units[u].prio] += (uint32_t)L.units[u].nitems;
+    }
+    std::vector<uint32_t> item_unit(L.nitems);
+    for (uint32_t u = 0; u < nu; u++)
+        for (int i = 0; i < L.units[u].nitems; i++)
+            item_unit[L.units[u].item_base + i] = u;
+
+    if (mode == MODE_GRAPH) {
+        e->preds.assign(nu, {});
+        for (uint32_t u = 0; u < nu; u++)
+            for (uint32_t v : L.succ[u])
+                e->preds[v].push_back(u);
+        HIPCHK(hipGraphCreate(&e->graph, 0));
+        std::vector<hipGraphNode_t> node(nu, nullptr);
+        for (uint32_t u = 0; u < nu; u++) {
+            UnitDesc d = L.units[u];
+            void *args[] = {&d};
+            hipKernelNodeParams kp;
+            std::memset(&kp, 0, sizeof kp);
+            kp.func = reinterpret_cast<void *>(k_unit);
+            kp.gridDim = dim3(d.nitems);
+            kp.blockDim = dim3(WG_THREADS);
+            kp.kernelParams = args;
+            std::vector<hipGraphNode_t> deps;
+            for (uint32_t p : e->preds[u])
+                deps.push_back(node[p]);
+            HIPCHK(hipGraphAddKernelNode(&node[u], e->graph, deps.data(), deps.size(), &kp));
+        }
+        HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+        *out = e;
+        return 0;
+    }
+    if (mode == MODE_STREAM) {
+        *out = e;
+        return 0;
+    }
+
+    // persistent mode: one device allocation for descriptors and state
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = align_up(off, 256);
+        off = o + bytes;
+        return o;
+    };
+    const size_t o_units = take(sizeof(UnitDesc) * nu);
+    const size_t o_iu = take(sizeof(uint32_t) * L.nitems);
+    const size_t o_succ = take(sizeof(uint32_t) * std::max<size_t>(succ.size(), 1));
+    const size_t o_dep = take(sizeof(int32_t) * nu);
+    const size_t o_left = take(sizeof(int32_t) * nu);
+    uint32_t total = 0;
+    for (int p = 0; p < NPRIO; p++)
+        total += cap[p];
+    const size_t o_slots = take(sizeof(uint32_t) * std::max<uint32_t>(total, 1));
+    const size_t o_init = take(sizeof(uint32_t) * std::max<uint32_t>(total, 1));
+    const size_t o_st = take(sizeof(SchedState));
+    const size_t o_trace = eopts.trace ? take(sizeof(uint64_t) * 2 * L.nitems) : 0;
+    e->dbytes = off;
+    HIPCHK(hipMalloc(&e->dbuf, e->dbytes));
+
+    // initial ring contents: items of the units without dependencies
+    std::vector<uint32_t> init(std::max<uint32_t>(total, 1), 0);
+    uint32_t ring_base[NPRIO], fill[NPRIO] = {0, 0, 0, 0};
+    for (int p = 0, b = 0; p < NPRIO; p++) {
+        ring_base[p] = b;
+        b += cap[p];
+    }
+    for (uint32_t u = 0; u < nu; u++) {
+        const UnitDesc &d = L.units[u];
+        if (d.ndeps == 0)
+            for (int i = 0; i < d.nitems; i++)
+                init[ring_base[d.prio] + fill[d.prio]++] = d.item_base + i + 1;
+    }
+    HIPCHK(hipMemcpy(e->dbuf + o_units, L.units.data(), sizeof(UnitDesc) * nu,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->dbuf + o_iu, item_unit.data(), sizeof(uint32_t) * L.nitems,
+                     hipMemcpyHostToDevice));
+    if (!succ.empty())
+        HIPCHK(hipMemcpy(e->dbuf + o_succ, succ.data(), sizeof(uint32_t) * succ.size(),
+                         hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->dbuf + o_init, init.data(), sizeof(uint32_t) * init.size(),
+                     hipMemcpyHostToDevice));
+
+    ExecView &ev = e->ev;
+    ev.units = reinterpret_cast<const UnitDesc *>(e->dbuf + o_units);
+    ev.item_unit = reinterpret_cast<const uint32_t *>(e->dbuf + o_iu);
+    ev.succ = reinterpret_cast<const uint32_t *>(e->dbuf + o_succ);
+    ev.dep_count = reinterpret_cast<int32_t *>(e->dbuf + o_dep);
+    ev.items_left = reinterpret_cast<int32_t *>(e->dbuf + o_left);
+    for (int p = 0; p < NPRIO; p++) {
+        ev.slots[p] = reinterpret_cast<uint32_t *>(e->dbuf + o_slots) + ring_base[p];
+        ev.cap[p] = cap[p];
+    }
+    e->slots_init = reinterpret_cast<uint32_t *>(e->dbuf + o_init);
+    e->slots_total = total;
+    ev.st = reinterpret_cast<SchedState *>(e->dbuf + o_st);
+    ev.nunits = nu;
+    ev.nitems = (uint32_t)L.nitems;
+    e->trace = eopts.trace ? reinterpret_cast<uint64_t *>(e->dbuf + o_trace) : nullptr;
+    ev.trace = e->trace;
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device));
+    e->wall_khz = khz > 0 ? (uint64_t)khz : 100000;
+    ev.timeout_ticks = (uint64_t)(eopts.timeout_s * 1000.0 * (double)e->wall_khz);
+
+    int ncu = 0, per_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, e->device));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, WG_THREADS, 0));
+    if (per_cu < 1)
+        per_cu = 1;
+    e->grid = eopts.grid > 0 ? eopts.grid : ncu * per_cu;
+    // the tail values the reset kernel restores before every run
+    std::memcpy(e->init_tail, fill, sizeof fill);
+    *out = e;
+    return 0;
+}
+
+int exec_launch(Exec *e, hipStream_t s, std::string &err)
+{
+    if (e->mode == MODE_GRAPH) {
+        HIPCHK(hipGraphLaunch(e->gexec, s));
+        return 0;
+    }
+    if (e->mode == MODE_STREAM)
+        return run_units(e->low, s, err);
+    ResetArgs ra;
+    std::memcpy(ra.init_tail, e->init_tail, sizeof ra.init_tail);
+    ra.slots_init = e->slots_init;
+    ra.slots_flat = e->ev.slots[0];
+    ra.slots_total = e->slots_total;
+    const size_t work = std::max<size_t>(e->ev.nunits, e->slots_total);
+    const int rg = (int)std::min<size_t>(1024, (work + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_reset, dim3(rg), dim3(256), 0, s, e->ev, ra);
+    HIPCHK(hipGetLastError());
+    if (e->ev.nitems == 0)
+        return 0;
+    hipLaunchKernelGGL(k_persistent, dim3(e->grid), dim3(WG_THREADS), 0, s, e->ev);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int exec_check(Exec *e, std::string &err)
+{
+    if (e->mode != MODE_PERSISTENT)
+        return 0;
+    SchedState st;
+    HIPCHK(hipMemcpy(&st, e->ev.st, sizeof st, hipMemcpyDeviceToHost));
+    if (st.err || st.done != e->ev.nitems) {
+        err = "persistent scheduler stopped after " + std::to_string(st.done) + " of " +
+              std::to_string(e->ev.nitems) + " items (no progress for the time-out)";
+        return -1;
+    }
+    return 0;
+}
+
+int exec_trace(Exec *e, std::vector<uint64_t> &stamps, std::string &err)
+{
+    if (!e->trace) {
+        err = "tracing was not enabled";
+        return -1;
+    }
+    stamps.resize(2 * (size_t)e->ev.nitems);
+    HIPCHK(hipMemcpy(stamps.data(), e->trace, sizeof(uint64_t) * stamps.size(),
+                     hipMemcpyDeviceToHost));
+    return 0;
+}
+
+void exec_destroy(Exec *e)
+{
+    if (!e)
+        return;
+    if (e->gexec)
+        (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph)
+        (void)hipGraphDestroy(e->graph);
+    if (e->dbuf)
+        (void)hipFree(e->dbuf);
+    delete e;
+}
+
+// ---------------------------------------------------------------------------
+// generators, residuals and small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t sm64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ void k_gen_uniform(double *M, size_t count, uint64_t seed, double lo, double hi)
+{
+    const uint64_t s0 = sm64(seed);
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t z = sm64(s0 + k);
+        M[k] = lo + (hi - lo) * ((double)(z >> 11) * (1.0 / 9007199254740992.0));
+    }
+}
+
+int gen_uniform_device(double *M, size_t count, uint64_t seed, double lo, double hi,
+                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gen_uniform, dim3(2048), dim3(256), 0, s, M, count, seed, lo, hi);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// A <- A / div + shift I
+__global__ void k_add_diag_scale(double *A, int n, size_t lda, double div, double shift)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % n, j = k / n;
+        double *p = A + i + j * lda;
+        *p = *p / div + (i == j ? shift : 0.0);
+    }
+}
+
+int add_diag_scale(double *A, int n, int lda, double div, double shift, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_add_diag_scale, dim3(2048), dim3(256), 0, s, A, n, (size_t)lda, div,
+                       shift);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_symmetrize_lower(double *A, int n, size_t lda)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % n, j = k / n;
+        if (i < j)
+            A[i + j * lda] = A[j + i * lda];
+    }
+}
+
+int symmetrize_lower(double *A, int n, int lda, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_symmetrize_lower, dim3(2048), dim3(256), 0, s, A, n, (size_t)lda);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_fill_int(int32_t *p, size_t count, int32_t v)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count;
+         k += (size_t)gridDim.x * blockDim.x)
+        p[k] = v;
+}
+
+int fill_int(int32_t *p, size_t count, int32_t v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_int, dim3(64), dim3(256), 0, s, p, count, v);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// F packed (LU or Cholesky) -> explicit L (unit lower for LU) and U (upper
+// for LU; unused for Cholesky).
+__global__ void k_split_lu(const double *F, int n, size_t ldf, double *L, double *U, size_t ld,
+                           int chol)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % n, j = k / n;
+        const double f = F[i + j * ldf];
+        if (chol) {
+            L[i + j * ld] = i >= j ? f : 0.0;
+        } else {
+            L[i + j * ld] = i > j ? f : (i == j ? 1.0 : 0.0);
+            U[i + j * ld] = i <= j ? f : 0.0;
+        }
+    }
+}
+
+int split_lu(const double *F, int n, int ldf, double *L, double *U, int ld, int chol,
+             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_split_lu, dim3(4096), dim3(256), 0, s, F, n, (size_t)ldf, L, U,
+                       (size_t)ld, chol);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_sub_frob2(const double *A, size_t lda, const double *B, size_t ldb, int m,
+                            int n, double *acc)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)m * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % m, j = k / m;
+        const double d = B ? A[i + j * lda] - B[i + j * ldb] : A[i + j * lda];
+        s += d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicAdd(acc, red[0]);
+}
+
+int sub_frob2(const double *A, int lda, const double *B, int ldb, int m, int n, double *acc,
+              hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sub_frob2, dim3(2048), dim3(256), 0, s, A, (size_t)lda, B, (size_t)ldb,
+                       m, n, acc);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_copy_matrix(const double *src, size_t lds, double *dst, size_t ldd, int m,
+                              int n)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)m * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % m, j = k / m;
+        dst[i + j * ldd] = src[i + j * lds];
+    }
+}
+
+int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_matrix, dim3(4096), dim3(256), 0, s, src, (size_t)lds, dst,
+                       (size_t)ldd, m, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace mu

@@ -1,0 +1,67 @@
+// mu_runtime.hpp -- execution of lowered unit DAGs on one gfx950 GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "mu_dag.hpp"
+
+namespace mu {
+
+enum Mode : int {
+    MODE_GRAPH = 0,       // one hipGraph kernel node per unit, explicit edges ("--tiled")
+    MODE_PERSISTENT = 1,  // one persistent scheduler kernel ("--subgraph")
+    MODE_STREAM = 2,      // units launched in topological order on the stream
+};
+
+struct ExecOpts {
+    int grid = 0;              // persistent grid; 0 = one workgroup per resident slot
+    double timeout_s = 30.0;   // persistent: seconds without progress before abort
+    int trace = 0;             // persistent: record per-item wall-clock stamps
+};
+
+struct Exec {
+    int mode = MODE_PERSISTENT;
+    int device = 0;
+    Lowered low;
+    // device memory
+    char *dbuf = nullptr;
+    size_t dbytes = 0;
+    ExecView ev{};
+    uint32_t *slots_init = nullptr;  // initial queue contents, same layout as the rings
+    uint32_t slots_total = 0;
+    uint32_t init_tail[NPRIO] = {0, 0, 0, 0};
+    uint64_t *trace = nullptr;
+    int grid = 0;
+    uint64_t wall_khz = 100000;
+    // graph mode
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<std::vector<uint32_t>> preds;
+};
+
+int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts &eopts,
+                Exec **out, std::string &err);
+int exec_launch(Exec *e, hipStream_t s, std::string &err);
+// After the stream has finished: scheduler status (0 = fine).
+int exec_check(Exec *e, std::string &err);
+int exec_trace(Exec *e, std::vector<uint64_t> &stamps, std::string &err);
+void exec_destroy(Exec *e);
+
+// Eager execution of the units of a lowered DAG in topological order.
+int run_units(const Lowered &low, hipStream_t s, std::string &err);
+
+// Device helpers used by the C ABI.
+int gen_uniform_device(double *M, size_t count, uint64_t seed, double lo, double hi,
+                       hipStream_t s);
+int add_diag_scale(double *A, int n, int lda, double div, double shift, hipStream_t s);
+int symmetrize_lower(double *A, int n, int lda, hipStream_t s);
+int fill_int(int32_t *p, size_t count, int32_t v, hipStream_t s);
+int split_lu(const double *F, int n, int ldf, double *L, double *U, int ld, int chol,
+             hipStream_t s);
+int sub_frob2(const double *A, int lda, const double *B, int ldb, int m, int n, double *acc,
+              hipStream_t s);
+int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, hipStream_t s);
+
+}  // namespace mu

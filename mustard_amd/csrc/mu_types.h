@@ -1,0 +1,97 @@
+// mu_types.h -- descriptors shared by the host runtime and the gfx950 kernels.
+//
+// A recorded tile task (the unit the reference captures as one cuBLAS /
+// cuSOLVER call, mustard/lu_mustard.cu:343-535) is lowered to one or more
+// *units*.  A unit is a flat set of independent work items (one workgroup
+// each); units form a DAG through dependency counters and successor lists.
+// Large diagonal factorizations and large triangular solves lower to a nested
+// DAG of units over sub-tiles, the way the reference nests a tiled LU for
+// tiles above MAX_TILE (mustard/lu_mustard.cu:133-254, mustard/mustard.h:214-248).
+#pragma once
+#include <stdint.h>
+
+namespace mu {
+
+// Unit operations.
+enum Op : int32_t {
+    OP_NONE = 0,
+    OP_GEMM = 1,   // C <- beta C + alpha op(A) op(B); items = bm x bn output blocks
+    OP_SYRK = 2,   // lower(C) <- beta C + alpha A A^T; items = lower output blocks
+    OP_TRSM = 3,   // panel triangular solve T Y = Z; items = panels of <= 16 right-hand sides
+    OP_GETRF = 4,  // no-pivot LU of an n <= SMALL_MAX tile in LDS; one item
+    OP_POTRF = 5,  // lower Cholesky of an n <= SMALL_MAX tile in LDS; one item
+};
+
+// Unit flags.
+enum Flag : int32_t {
+    F_TA = 1 << 0,       // GEMM: op(A) = A^T
+    F_TB = 1 << 1,       // GEMM: op(B) = B^T
+    F_VEC = 1 << 2,      // operands 16-byte aligned with even leading dims: vector loads
+    F_T_TRANS = 1 << 3,  // TRSM: triangle read transposed, T[r][c] = t[c + r*ldt]
+    F_Z_ROWS = 1 << 4,   // TRSM: right-hand sides are rows, Z[r][c] = z[c + r*ldz]
+    F_UNIT = 1 << 5,     // TRSM: unit diagonal
+    F_LOW_F32 = 1 << 6,  // GEMM/SYRK: operands rounded to FP32, FP32 MFMA, FP32 accumulation
+    F_LOW_BF16 = 1 << 7, // GEMM/SYRK: operands rounded to BF16, FP32 accumulation
+    F_REV = 1 << 8,      // TRSM: row order of T and Z reversed (upper-triangular solves)
+};
+
+// Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).
+constexpr int WG_THREADS = 256;
+constexpr int GEMM_BM = 128;
+constexpr int GEMM_BN = 128;
+constexpr int GEMM_BK = 16;
+constexpr int TRSM_W = 16;        // right-hand sides per TRSM item
+constexpr int TRSM_NMAX = 1024;   // largest triangle a TRSM item keeps in LDS
+constexpr int SMALL_MAX = 128;    // largest tile a GETRF/POTRF item factors in LDS
+constexpr int NPRIO = 4;          // priority levels of the device queue
+
+struct UnitDesc {
+    int32_t op;
+    int32_t flags;
+    int32_t m, n, k;       // operation sizes
+    int32_t lda, ldb, ldc; // leading dimensions
+    int32_t gm, gn;        // item grid (GEMM: blocks; TRSM: gn panels)
+    int32_t nitems;
+    int32_t prio;          // 0 = most urgent
+    uint32_t item_base;    // global id of item 0
+    uint32_t succ_begin;   // successors: succ[succ_begin .. succ_end)
+    uint32_t succ_end;
+    int32_t ndeps;         // initial dependency count
+    int32_t info_base;     // global column offset reported by factor items
+    int32_t task;          // recorded task this unit belongs to
+    double alpha, beta;
+    const double *a;
+    const double *b;
+    double *c;
+    int32_t *info;         // LAPACK-style info cell (atomic min of failing column), or null
+    int32_t pad_[2];
+};
+static_assert(sizeof(UnitDesc) == 128, "UnitDesc must stay 128 bytes");
+
+// Per-run mutable scheduler state, all in device memory.
+struct SchedState {
+    uint32_t head[NPRIO];
+    uint32_t tail[NPRIO];
+    uint32_t done;        // completed items
+    uint32_t abort;       // set on time-out; every workgroup exits
+    uint32_t err;         // error code (1 = time-out)
+    uint32_t pad_[5];
+};
+
+// Everything the device scheduler needs, passed by value to the kernels.
+struct ExecView {
+    const UnitDesc *units;
+    const uint32_t *item_unit;     // item id -> unit id
+    const uint32_t *succ;          // successor unit ids (CSR)
+    int32_t *dep_count;            // per unit, reset each run
+    int32_t *items_left;           // per unit, reset each run
+    uint32_t *slots[NPRIO];        // queue rings: item id + 1, 0 = empty
+    uint32_t cap[NPRIO];
+    SchedState *st;
+    uint32_t nunits;
+    uint32_t nitems;
+    uint64_t timeout_ticks;        // wall-clock ticks without progress before abort
+    uint64_t *trace;               // optional: per item (start, end) wall-clock stamps
+};
+
+}  // namespace mu

@@ -1,0 +1,196 @@
+"""Parity of the gfx950 tile kernels against LAPACK semantics.
+
+GEMM/SYRK are checked against numpy's float64 product (a dgemm), TRSM against
+scipy's solve_triangular, and GETRF/POTRF against the oracle (LAPACK dpotrf,
+and the no-pivot Doolittle restatement, itself pinned to dgetrf with identity
+pivots in test_oracle.py).  Every call goes through the C ABI.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from tests.util import EPS, close_rel, diag_dominant, rand, spd
+
+pytestmark = pytest.mark.gpu
+
+
+def _mu():
+    import mustard_amd as mu
+    return mu
+
+
+def _dev(A):
+    return _mu().to_device(np.asfortranarray(A))
+
+
+def _host(T, shape):
+    return _mu().to_host(T)[: shape[0], : shape[1]]
+
+
+GEMM_CASES = [
+    # m, n, k, transa, transb, pad, alpha, beta
+    (1, 1, 1, "N", "N", 0, -1.0, 1.0),
+    (16, 16, 4, "N", "N", 0, -1.0, 1.0),
+    (17, 33, 5, "N", "T", 0, -1.0, 1.0),
+    (128, 128, 16, "N", "N", 0, 1.0, 0.0),
+    (129, 130, 131, "T", "N", 0, 0.5, 2.0),
+    (300, 257, 1000, "N", "N", 3, -1.0, 1.0),
+    (257, 300, 999, "T", "T", 1, -1.0, 1.0),
+    (1000, 1000, 1000, "N", "N", 24, -1.0, 1.0),   # LU trailing update shape
+    (1000, 1000, 1000, "N", "T", 24, -1.0, 1.0),   # Cholesky trailing update shape
+    (125, 125, 125, "N", "N", 875, -1.0, 1.0),     # nested sub-tile, odd offsets
+    (64, 16, 1000, "N", "N", 0, -1.0, 1.0),
+]
+
+
+@pytest.mark.parametrize("m,n,k,ta,tb,pad,alpha,beta", GEMM_CASES)
+def test_dgemm(torch_gpu, m, n, k, ta, tb, pad, alpha, beta):
+    mu = _mu()
+    rng = np.random.default_rng(m * 7 + n * 13 + k)
+    ar, ac = (k, m) if ta == "T" else (m, k)
+    br, bc = (n, k) if tb == "T" else (k, n)
+    lda, ldb, ldc = ar + pad, br + pad, m + pad
+    A = rand(rng, lda, ac)
+    B = rand(rng, ldb, bc)
+    C = rand(rng, ldc, n)
+    if beta == 0.0:
+        C[:m, :n] = np.nan  # beta = 0 must not read C
+    dA, dB, dC = _dev(A), _dev(B), _dev(C)
+    mu.dgemm(ta, tb, m, n, k, alpha, mu.ptr(dA), lda, mu.ptr(dB), ldb, beta, mu.ptr(dC), ldc)
+    torch_gpu.cuda.synchronize()
+    opA = A[:ar, :ac].T if ta == "T" else A[:ar, :ac]
+    opB = B[:br, :bc].T if tb == "T" else B[:br, :bc]
+    ref = alpha * (opA @ opB) + (0.0 if beta == 0.0 else beta * C[:m, :n])
+    got = mu.to_host(dC)
+    # elementwise floating-point bound of a length-k dot product, for both sides
+    tol = 2 * (k + 2) * EPS * (abs(alpha) * (np.abs(opA) @ np.abs(opB)) + (
+        0.0 if beta == 0.0 else abs(beta) * np.abs(C[:m, :n]))) + 1e-300
+    assert np.all(np.abs(got[:m, :n] - ref) <= tol)
+    # nothing outside the m x n block may change
+    if pad:
+        assert np.array_equal(got[m:, :], C[m:, :])
+
+
+@pytest.mark.parametrize("n,k,pad", [(1, 1, 0), (17, 3, 0), (128, 64, 0), (129, 129, 1),
+                                     (300, 1000, 2), (1000, 1000, 24)])
+def test_dsyrk_lower(torch_gpu, n, k, pad):
+    mu = _mu()
+    rng = np.random.default_rng(n + 31 * k)
+    lda, ldc = n + pad, n + pad
+    A = rand(rng, lda, k)
+    C = rand(rng, ldc, n)
+    dA, dC = _dev(A), _dev(C)
+    mu.dsyrk("L", "N", n, k, -1.0, mu.ptr(dA), lda, 1.0, mu.ptr(dC), ldc)
+    torch_gpu.cuda.synchronize()
+    got = mu.to_host(dC)
+    ref = C[:n, :n] - A[:n, :] @ A[:n, :].T
+    lo = np.tril(np.ones((n, n), dtype=bool))
+    tol = 2 * (k + 2) * EPS * (np.abs(A[:n, :]) @ np.abs(A[:n, :]).T + np.abs(C[:n, :n]))
+    assert np.all(np.abs(got[:n, :n][lo] - ref[lo]) <= tol[lo])
+    # strict upper triangle untouched (the reference's dsyrk writes LOWER only)
+    assert np.array_equal(got[:n, :n][~lo], C[:n, :n][~lo])
+
+
+def _tri(rng, n, lower, unit):
+    T = rand(rng, n, n, -0.5, 0.5) / max(n, 1) ** 0.5
+    T[np.arange(n), np.arange(n)] = 1.0 + rng.uniform(0.5, 1.5, n)
+    mask = np.tril(np.ones((n, n), dtype=bool)) if lower else np.triu(np.ones((n, n), dtype=bool))
+    T = np.where(mask, T, np.nan)  # the other triangle must never be read
+    if unit:
+        T[np.arange(n), np.arange(n)] = np.nan  # nor the diagonal of a unit triangle
+    return np.asfortranarray(T)
+
+
+TRSM_SIZES = [(1, 1), (16, 16), (17, 5), (100, 37), (37, 100), (1000, 64), (64, 1000),
+              (1100, 20), (20, 1100)]
+
+
+@pytest.mark.parametrize("side", ["L", "R"])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("diag", ["U", "N"])
+@pytest.mark.parametrize("m,n", TRSM_SIZES)
+def test_dtrsm(torch_gpu, side, uplo, trans, diag, m, n):
+    mu = _mu()
+    rng = np.random.default_rng(m * 1000 + n + ord(side) + 2 * ord(uplo) + 3 * ord(trans))
+    na = m if side == "L" else n
+    T = _tri(rng, na, uplo == "L", diag == "U")
+    B = rand(rng, m, n)
+    alpha = 1.0 if (m + n) % 2 else 0.75
+    dT, dB = _dev(T), _dev(B)
+    mu.dtrsm(side, uplo, trans, diag, m, n, alpha, mu.ptr(dT), na, mu.ptr(dB), m)
+    torch_gpu.cuda.synchronize()
+    got = mu.to_host(dB)
+    Tc = np.where(np.isnan(T), 0.0, T)
+    if diag == "U":
+        Tc[np.arange(na), np.arange(na)] = 1.0
+    opT = Tc.T if trans == "T" else Tc
+    if side == "L":
+        ref = sl.solve_triangular(opT, alpha * B, lower=(uplo == "L") != (trans == "T"))
+    else:
+        ref = sl.solve_triangular(opT.T, alpha * B.T, lower=(uplo == "L") == (trans == "T")).T
+    assert np.all(np.isfinite(got))
+    assert close_rel(got, ref, rtol=1e-11) <= 1.0
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 128, 129, 250, 1000])
+def test_dgetrf_nopiv(torch_gpu, oracle, n):
+    mu = _mu()
+    rng = np.random.default_rng(n)
+    A = diag_dominant(rng, n)
+    ref = A.copy(order="F")
+    assert oracle.getrf_nopiv(ref) == 0
+    lu, piv = sl.lu_factor(A)
+    assert np.array_equal(piv, np.arange(n))
+    dA = _dev(A)
+    info = mu.info_cell()
+    mu.dgetrf_nopiv(n, mu.ptr(dA), n, mu.ptr(info))
+    torch_gpu.cuda.synchronize()
+    got = mu.to_host(dA)
+    assert mu.info_value(info) == 0
+    assert close_rel(got, ref, rtol=1e-11) <= 1.0
+    assert close_rel(got, lu, rtol=1e-11) <= 1.0
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 64, 128, 129, 250, 1000])
+def test_dpotrf(torch_gpu, oracle, n):
+    mu = _mu()
+    rng = np.random.default_rng(n + 5)
+    A = spd(rng, n)
+    A[np.triu_indices(n, 1)] = 7.0  # upper triangle must be left as it is
+    ref = A.copy(order="F")
+    assert oracle.potrf_lower(ref) == 0
+    dA = _dev(A)
+    info = mu.info_cell()
+    mu.dpotrf("L", n, mu.ptr(dA), n, mu.ptr(info))
+    torch_gpu.cuda.synchronize()
+    got = mu.to_host(dA)
+    assert mu.info_value(info) == 0
+    lo = np.tril(np.ones((n, n), dtype=bool))
+    assert close_rel(got[lo], ref[lo], rtol=1e-11) <= 1.0
+    assert np.array_equal(got[~lo], A[~lo])
+
+
+@pytest.mark.parametrize("n,col", [(16, 0), (64, 40), (300, 200)])
+def test_getrf_zero_pivot_info(torch_gpu, n, col):
+    """A zero pivot is reported as LAPACK info = first failing column (1-based)."""
+    mu = _mu()
+    A = np.asfortranarray(np.eye(n) * 2.0)
+    A[col, col] = 0.0
+    dA = _dev(A)
+    info = mu.info_cell()
+    mu.dgetrf_nopiv(n, mu.ptr(dA), n, mu.ptr(info))
+    torch_gpu.cuda.synchronize()
+    assert mu.info_value(info) == col + 1
+
+
+@pytest.mark.parametrize("n,col", [(16, 3), (64, 0), (300, 150)])
+def test_potrf_not_spd_info(torch_gpu, n, col):
+    mu = _mu()
+    A = np.asfortranarray(np.eye(n) * 4.0)
+    A[col, col] = -1.0
+    dA = _dev(A)
+    info = mu.info_cell()
+    mu.dpotrf("L", n, mu.ptr(dA), n, mu.ptr(info))
+    torch_gpu.cuda.synchronize()
+    assert mu.info_value(info) == col + 1

@@ -1,0 +1,165 @@
+"""Whole tiled factorizations on the GPU against the oracle.
+
+Each case records the reference's task sequence (lu_mustard.cu:343-535,
+cholesky_mustard.cu:221-400) through the C ABI and runs it in every execution
+mode.  Checks: elementwise agreement with the host LAPACK / oracle factors
+(1e-10 relative, floored at 1e-3 of the largest entry), the scaled residual
+||A - LU||_F / (||A||_F N eps) <= 10 computed on the GPU, and for N = 600 the
+reference's own verify line (sum |A - LU| <= 1e-6, include/verify.h:70).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from tests.util import close_rel, spd
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["subgraph", "tiled", "stream"]
+
+
+def _mu():
+    import mustard_amd as mu
+    return mu
+
+
+def _run(torch, kind, A, t, mode, **kw):
+    mu = _mu()
+    n = A.shape[0]
+    dA = mu.to_device(A)
+    info, stats = mu.factorize(kind, dA, n, t, mode=mode, **kw)
+    F = mu.to_host(dA)
+    res = mu.scaled_residual(mu.to_device(A), dA, n, chol=(kind == "chol"))
+    torch.cuda.synchronize()
+    return F, info, res, stats
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_config0_lu_600_t2(torch_gpu, oracle, mode):
+    """N=600 split in 2x2 tiles of 300, reference generator, --verify."""
+    mu = _mu()
+    A = mu.gen_ref_host(600)
+    assert np.array_equal(A, oracle.gen_ref(600))
+    F, info, res, _ = _run(torch_gpu, "lu", A, 2, mode)
+    assert info == 0
+    lu, piv = sl.lu_factor(A)
+    assert np.array_equal(piv, np.arange(600))
+    assert close_rel(F, lu) <= 1.0
+    assert res <= 10.0
+    assert mu.lib().mu_verify_lu_l1_host(A.ctypes.data, F.copy(order="F").ctypes.data,
+                                        600) <= 1e-6
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_cholesky_600_t2(torch_gpu, oracle, mode):
+    mu = _mu()
+    A = mu.gen_ref_host(600)
+    F, info, res, _ = _run(torch_gpu, "chol", A, 2, mode)
+    assert info == 0
+    L = sl.cholesky(A, lower=True)
+    lo = np.tril(np.ones(A.shape, dtype=bool))
+    assert close_rel(F[lo], L[lo]) <= 1.0
+    assert np.array_equal(F[~lo], A[~lo])  # upper triangle untouched
+    assert res <= 10.0
+    Fc = np.asfortranarray(F)
+    assert mu.lib().mu_verify_chol_l1_host(A.ctypes.data, Fc.ctypes.data, 600) <= 1e-6
+
+
+CASES = [
+    # n, t -- B = n / t; covers B = 1000 (nested GETRF/POTRF), ragged sub-tiles,
+    # B > 1024 (nested TRSM), T = 1 and B = 1
+    (3000, 3), (3000, 6), (3000, 12), (1000, 8), (130, 1), (8, 8), (4000, 2), (2100, 1),
+]
+
+
+@pytest.mark.parametrize("n,t", CASES)
+@pytest.mark.parametrize("kind", ["lu", "chol"])
+def test_tiled_vs_oracle(torch_gpu, oracle, kind, n, t):
+    rng = np.random.default_rng(n * 10 + t)
+    A = oracle.gen_ref(n) if kind == "lu" else spd(rng, n)
+    ref = A.copy(order="F")
+    if kind == "lu":
+        assert oracle.tiled_lu(ref, t) == 0
+    else:
+        assert oracle.tiled_cholesky(ref, t) == 0
+    for mode in (["subgraph", "tiled"] if n <= 3000 else ["subgraph"]):
+        F, info, res, _ = _run(torch_gpu, kind, A, t, mode)
+        assert info == 0
+        if kind == "lu":
+            assert close_rel(F, ref) <= 1.0, mode
+        else:
+            lo = np.tril(np.ones(A.shape, dtype=bool))
+            assert close_rel(F[lo], ref[lo]) <= 1.0, mode
+        assert res <= 10.0, mode
+
+
+@pytest.mark.parametrize("grid", [1, 3, 64])
+def test_persistent_small_grids(torch_gpu, oracle, grid):
+    """The scheduler is correct with any number of workgroups, down to one."""
+    A = oracle.gen_ref(1000)
+    ref = A.copy(order="F")
+    oracle.tiled_lu(ref, 4)
+    F, info, res, st = _run(torch_gpu, "lu", A, 4, "subgraph", grid=grid)
+    assert st["grid"] == grid
+    assert info == 0 and res <= 10.0
+    assert close_rel(F, ref) <= 1.0
+
+
+def test_persistent_trace(torch_gpu, oracle):
+    mu = _mu()
+    n, t = 1200, 4
+    A = oracle.gen_ref(n)
+    dA = mu.to_device(A)
+    info = mu.info_cell()
+    dag = mu.Dag().record_lu(mu.ptr(dA), n, t, None, mu.ptr(info))
+    ex = mu.Exec(dag, "subgraph", trace=True)
+    ex.launch()
+    torch_gpu.cuda.synchronize()
+    ex.check()
+    tr = ex.trace()
+    assert len(tr) == ex.stats()["items"]
+    assert all(e >= s > 0 for s, e, *_ in tr)
+    # an item starts only after every item of every predecessor task ended
+    first = {}
+    last = {}
+    for s, e, _, task, _ in tr:
+        first[task] = min(first.get(task, s), s)
+        last[task] = max(last.get(task, e), e)
+    for task in range(dag.num_tasks()):
+        for p in dag.deps(task):
+            assert last[p] <= first[task]
+
+
+def test_persistent_timeout_aborts_cleanly(torch_gpu, oracle):
+    """A zero time-out makes idle workgroups abort; every workgroup exits and
+    the run reports MU_ERUN instead of hanging."""
+    mu = _mu()
+    A = oracle.gen_ref(600)
+    dA = mu.to_device(A)
+    dag = mu.Dag().record_lu(mu.ptr(dA), 600, 6)
+    ex = mu.Exec(dag, "subgraph", grid=8, timeout_s=1e-9)
+    ex.launch()
+    torch_gpu.cuda.synchronize()
+    with pytest.raises(mu.MustardError) as e:
+        ex.check()
+    assert e.value.code == -4
+
+
+def test_repeated_launches_identical(torch_gpu, oracle):
+    """Re-running the same executable (counters and rings reset on the device)
+    gives bit-identical factors."""
+    mu = _mu()
+    n, t = 2000, 4
+    A = oracle.gen_ref(n)
+    dA = mu.to_device(A)
+    orig = dA.clone()
+    dag = mu.Dag().record_lu(mu.ptr(dA), n, t)
+    ex = mu.Exec(dag, "subgraph")
+    outs = []
+    for _ in range(3):
+        dA.copy_(orig)
+        ex.launch()
+        torch_gpu.cuda.synchronize()
+        ex.check()
+        outs.append(dA.clone())
+    assert all(bool((o == outs[0]).all()) for o in outs)
