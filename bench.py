@@ -1,0 +1,212 @@
+"""Benchmark of the north-star workload on one MI355X.
+
+Workload: tiled LU and tiled Cholesky, FP64, N = 24000 cut in 24 x 24 tiles of
+1000, each run as one hipGraph launch whose nodes are level-batched kernels
+(every independent task of one DAG level in one launch; the reference's
+--tiled mode).  LU uses the reference generator (include/gen.h, glibc
+rand, seed 420); Cholesky uses A = M M^T / N + N I with M uniform [-1, 1)
+(seed 420), built on the device.
+
+One step = one LU plus one Cholesky factorization of the resident matrices
+(each restored from a pristine device copy before it runs; the restore is
+outside the timed window, as in the reference, lu_mustard.cu:605-611).
+value = (2N^3/3 + N^3/3) / (LU time + Cholesky time), in GFLOP/s.
+
+Extra objects on the JSON line: `roofline` for the dominant kernel (k_batch,
+the level-batched tile kernel: every launch of one factorization timed with
+HIP events on the stream it runs on, in a separate pass) and
+`cpu_baseline` (the oracle's OpenMP task DAG over host BLAS, timed on a
+bounded sample on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LU/Cholesky GFLOP/s at N=24000 FP64 vs mixed+IR at equal backward error"
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 MFMA (public spec)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n=24000, t=24):
+    """Oracle OpenMP tiled LU + Cholesky on the host cores: the same N and tile
+    size as the GPU workload, both on the reference generator's matrix (it is
+    symmetric positive definite), roughly 10-20 s of CPU work."""
+    from oracle import oracle as O
+    O.lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, os.cpu_count() or 1, 16))
+    A = O.gen_ref(n)
+    F = A.copy(order="F")
+    t0 = time.perf_counter()
+    O.tiled_lu(F, t, threads=threads)
+    t_lu = time.perf_counter() - t0
+    F = A.copy(order="F")
+    t0 = time.perf_counter()
+    O.tiled_cholesky(F, t, threads=threads)
+    t_ch = time.perf_counter() - t0
+    flops = 2.0 * n ** 3 / 3 + n ** 3 / 3.0
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(flops / (t_lu + t_ch) * 1e-9, 1), "unit": "GFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"tiled LU + Cholesky, N={n}, {t}x{t} tiles of {n // t}, OpenMP task DAG "
+                      f"with single-threaded OpenBLAS per tile ({os.path.basename(O.BLAS_PATH or 'loops')}), "
+                      f"LU {t_lu:.2f} s, Cholesky {t_ch:.2f} s, CPU {cpu}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=24000)
+    ap.add_argument("--t", type=int, default=24)
+    ap.add_argument("--mode", default="tiled", choices=["tiled", "stream"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-residual", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build runs the tiled factorizations on a "
+              "single GPU (multi-GPU distribution is out of scope); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mustard_amd as mu
+
+    torch.cuda.set_device(0)
+    n, t = args.n, args.t
+    b = n // t
+    stream = torch.cuda.Stream()
+    log(f"[bench] generating N={n} matrices")
+    A_lu0 = mu.to_device(mu.gen_ref_host(n))
+    A_ch0 = mu.gen_spd(n, 420)
+    A_lu = torch.empty_like(A_lu0)
+    A_ch = torch.empty_like(A_ch0)
+    info = mu.info_cell()
+
+    execs = {}
+    with torch.cuda.stream(stream):
+        for kind, A in (("lu", A_lu), ("chol", A_ch)):
+            dag = mu.Dag()
+            if kind == "lu":
+                dag.record_lu(mu.ptr(A), n, t, None, mu.ptr(info))
+            else:
+                dag.record_cholesky(mu.ptr(A), n, t, None, mu.ptr(info))
+            execs[kind] = (dag, mu.Exec(dag, args.mode))
+    st = {k: e.stats() for k, (d, e) in execs.items()}
+    log(f"[bench] lowered: LU {st['lu']['units']} units / {st['lu']['items']} items / "
+        f"{st['lu']['levels']} levels, Cholesky {st['chol']['units']} units / "
+        f"{st['chol']['items']} items / {st['chol']['levels']} levels")
+
+    def run(kind):
+        src, dst = (A_lu0, A_lu) if kind == "lu" else (A_ch0, A_ch)
+        with torch.cuda.stream(stream):
+            dst.copy_(src)
+            info.fill_(2 ** 31 - 1)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            execs[kind][1].launch(stream)
+            e1.record(stream)
+        stream.synchronize()
+        execs[kind][1].check()
+        assert mu.info_value(info) == 0, f"{kind}: info {mu.info_value(info)}"
+        return e0.elapsed_time(e1)
+
+    for w in range(args.warmup):
+        tl, tc = run("lu"), run("chol")
+        log(f"[bench] warmup {w}: LU {tl:.1f} ms, Cholesky {tc:.1f} ms")
+    torch.cuda.synchronize()
+    lu_ms, ch_ms = [], []
+    wall0 = time.perf_counter()
+    for s in range(args.steps):
+        lu_ms.append(run("lu"))
+        ch_ms.append(run("chol"))
+        log(f"[bench] step {s}: LU {lu_ms[-1]:.1f} ms, Cholesky {ch_ms[-1]:.1f} ms")
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - wall0) / max(args.steps, 1) * 1e3
+
+    f_lu, f_ch = 2.0 * n ** 3 / 3.0, n ** 3 / 3.0
+    m_lu, m_ch = sum(lu_ms) / len(lu_ms), sum(ch_ms) / len(ch_ms)
+    value = (f_lu + f_ch) / ((m_lu + m_ch) * 1e-3) * 1e-9
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(m_lu + m_ch, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"tiled LU + tiled Cholesky, FP64, N={n}, {t}x{t} tiles of {b}, "
+                               f"one {'hipGraph of level-batched kernels' if args.mode == 'tiled' else 'stream of unit kernels'} each",
+                   "n": n, "tiles": t, "tile_size": b, "mode": args.mode,
+                   "lu_matrix": "reference generator (gen.h, glibc rand, seed 420)",
+                   "cholesky_matrix": "M M^T/N + N I, M uniform[-1,1) seed 420"},
+        "lu": {"ms": round(m_lu, 3), "ms_min": round(min(lu_ms), 3),
+               "gflops": round(f_lu / (m_lu * 1e-3) * 1e-9, 1),
+               "frac_fp64_peak": round(f_lu / (m_lu * 1e-3) * 1e-12 / PEAK_FP64_TFLOPS, 4)},
+        "cholesky": {"ms": round(m_ch, 3), "ms_min": round(min(ch_ms), 3),
+                     "gflops": round(f_ch / (m_ch * 1e-3) * 1e-9, 1),
+                     "gflops_ref_convention_2n3_3": round(2 * f_ch / (m_ch * 1e-3) * 1e-9, 1),
+                     "frac_fp64_peak": round(f_ch / (m_ch * 1e-3) * 1e-12 / PEAK_FP64_TFLOPS, 4)},
+        "wall_ms_per_step_incl_restore": round(wall, 3),
+    }
+    # Dominant kernel: k_batch.  Every level of each factorization is launched
+    # alone between two events on the bench stream; achieved = algorithmic
+    # flops of the factorizations / summed k_batch time (= average flops per
+    # launch / average launch duration).
+    if args.mode == "tiled":
+        launches, kern_ms = 0, 0.0
+        for kind in ("lu", "chol"):
+            src, dst = (A_lu0, A_lu) if kind == "lu" else (A_ch0, A_ch)
+            ex = execs[kind][1]
+            nlev = ex.stats()["levels"]
+            with torch.cuda.stream(stream):
+                dst.copy_(src)
+                info.fill_(2 ** 31 - 1)
+                evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * nlev)]
+                for l in range(nlev):
+                    evs[2 * l].record(stream)
+                    ex.launch_level(l, stream)
+                    evs[2 * l + 1].record(stream)
+            stream.synchronize()
+            assert mu.info_value(info) == 0
+            kern_ms += sum(evs[2 * l].elapsed_time(evs[2 * l + 1]) for l in range(nlev))
+            launches += nlev
+        achieved = (f_lu + f_ch) / (kern_ms * 1e-3) * 1e-12
+        out["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3),
+                           "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                           "kernel": "k_batch", "launches": launches,
+                           "avg_launch_ms": round(kern_ms / launches, 4),
+                           "avg_launch_gflop": round((f_lu + f_ch) / launches * 1e-9, 3)}
+        log(f"[bench] k_batch: {launches} launches, {kern_ms:.1f} ms total, "
+            f"{achieved:.2f} TFLOP/s")
+    if not args.no_residual:
+        for kind, A0, A in (("lu", A_lu0, A_lu), ("cholesky", A_ch0, A_ch)):
+            r = mu.scaled_residual(A0, A, n, chol=(kind == "cholesky"))
+            out[kind]["scaled_residual"] = round(r, 4)
+            log(f"[bench] {kind} scaled residual {r:.4f}")
+    if not args.no_cpu:
+        log("[bench] CPU baseline")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

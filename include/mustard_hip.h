@@ -29,8 +29,9 @@ extern "C" {
 #define MU_ENOTSUP (-5) /* combination not supported */
 
 /* Execution modes of a recorded DAG. */
-#define MU_MODE_TILED 0    /* one hipGraph, one kernel node per unit (reference --tiled) */
-#define MU_MODE_SUBGRAPH 1 /* one persistent scheduler kernel (reference --subgraph) */
+#define MU_MODE_TILED 0    /* one hipGraph of batched kernels, one per DAG level (reference --tiled) */
+#define MU_MODE_SUBGRAPH 1 /* one persistent scheduler kernel (reference --subgraph); switched
+                              off in this build: mu_exec_create returns MU_ENOTSUP */
 #define MU_MODE_STREAM 2   /* units launched one by one on the stream (debugging) */
 
 /* Precision of trailing GEMM/SYRK updates. */
@@ -59,6 +60,7 @@ typedef struct {
     double critical_path_us;  /* scheduler's estimate */
     int priority_units[4];    /* units per priority level */
     double wall_clock_khz;
+    int levels;               /* tiled mode: kernel launches (DAG levels) per run */
 } mu_exec_stats_t;
 
 /* ---- errors and devices ------------------------------------------------ */
@@ -134,6 +136,17 @@ int mu_dag_task_deps(mu_dag_t dag, int task, int *deps, int max, int *ndeps);
 int mu_dag_add_edge(mu_dag_t dag, int src, int dst);
 /* Problem tag for batched recording: tiles of different tags never alias. */
 int mu_dag_set_tag(mu_dag_t dag, int tag);
+/* Operation codes reported by mu_dag_task_info. */
+#define MU_OP_GEMM 1
+#define MU_OP_SYRK 2
+#define MU_OP_TRSM 3
+#define MU_OP_GETRF 4
+#define MU_OP_POTRF 5
+/* Written tile and operation codes of a recorded task (for DOT output and
+ * inspection; the reference prints its graphs with cudaGraphDebugDotPrint,
+ * lu_mustard.cu:583-585, 634-635). */
+int mu_dag_task_info(mu_dag_t dag, int task, int *w_row, int *w_col, int *ops, int max_ops,
+                     int *nops);
 
 /* ---- executables ---------------------------------------------------------
  * mu_exec_create lowers the DAG and uploads it (the reference's graph
@@ -142,6 +155,10 @@ int mu_dag_set_tag(mu_dag_t dag, int tag);
  * 640).  The recorded device pointers must stay valid. */
 int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t *exec);
 int mu_exec_launch(mu_exec_t exec, void *stream);
+/* Tiled executables: launch the batched kernel of one DAG level alone, and
+ * its item count (per-launch timing; a full run is every level in order). */
+int mu_exec_launch_level(mu_exec_t exec, int level, void *stream);
+int mu_exec_level_items(mu_exec_t exec, int level, long long *items);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);
 int mu_exec_stats(mu_exec_t exec, mu_exec_stats_t *stats);
@@ -179,6 +196,21 @@ double mu_verify_chol_l1_host(const double *A, const double *F, int n);
  * are n x n with leading dimension lda; `work` is 3 n*n doubles of scratch. */
 int mu_scaled_residual(const double *A, const double *F, int n, int lda, int chol,
                        double *work, double *result, void *stream);
+/* The same product, returning ||A - LU||_F, ||A||_F and the reference's
+ * verify quantity sum |A - LU| (include/verify.h:46-53), all on the device. */
+int mu_residual_norms(const double *A, const double *F, int n, int lda, int chol, double *work,
+                      double *frob_res, double *frob_a, double *l1_res, void *stream);
+
+/* ---- diagnostics ----------------------------------------------------------
+ * Run the persistent scheduler's queue/counter protocol for `dag` on host
+ * threads (no GPU; items are checked, not computed).  results[0..5] =
+ * {items executed once, duplicate executions, dependency-order violations,
+ * scheduler error (0 ok, 1 time-out), items counted done, total items}.
+ * `bump_unit` >= 0 adds a never-satisfied dependency to that unit (tests the
+ * progress watchdog).  No reference counterpart: the reference has no host
+ * model of its device queue (mustard/broker_queue.h). */
+int mu_dag_emulate(mu_dag_t dag, int threads, double timeout_s, int bump_unit,
+                   long long *results);
 
 #ifdef __cplusplus
 }

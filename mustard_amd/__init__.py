@@ -3,8 +3,9 @@
 Python mirror of the reference's single-GPU path: a task DAG recorded with
 the TiledGraphCreator rule (mustard/mustard.h:132-384) in the order of
 tiledLU / tiledCholesky (mustard/lu_mustard.cu:343-535,
-mustard/cholesky_mustard.cu:221-400), executed either as one hipGraph
-(``--tiled``) or by the persistent device scheduler (``--subgraph``).
+mustard/cholesky_mustard.cu:221-400), executed as one hipGraph of
+level-batched kernels (``--tiled``); the persistent device scheduler
+(``--subgraph``) is switched off in this build.
 
 Device memory and streams come from PyTorch-ROCm; all arithmetic runs in
 ``libmustard_hip.so`` (hand-written gfx950 kernels).  Column-major matrices are
@@ -156,7 +157,7 @@ class Dag:
 class Exec:
     """A lowered, uploaded DAG ready to launch."""
 
-    def __init__(self, dag: Dag, mode=MODE_SUBGRAPH, grid=0, timeout_s=30.0, trace=False,
+    def __init__(self, dag: Dag, mode=MODE_TILED, grid=0, timeout_s=30.0, trace=False,
                  small_max=0, trsm_max=0):
         if isinstance(mode, str):
             mode = MODES[mode]
@@ -171,12 +172,21 @@ class Exec:
     def check(self):
         call("mu_exec_check", self.h)
 
+    def launch_level(self, level, stream=None):
+        call("mu_exec_launch_level", self.h, level, stream_handle(stream))
+
+    def level_items(self, level):
+        n = ctypes.c_longlong()
+        call("mu_exec_level_items", self.h, level, ctypes.byref(n))
+        return n.value
+
     def stats(self):
         s = ExecStats()
         call("mu_exec_stats", self.h, ctypes.byref(s))
         return {"mode": s.mode, "grid": s.grid, "units": s.units, "items": s.items,
                 "edges": s.edges, "flops": s.flops, "critical_path_us": s.critical_path_us,
-                "priority_units": list(s.priority_units), "wall_clock_khz": s.wall_clock_khz}
+                "priority_units": list(s.priority_units), "wall_clock_khz": s.wall_clock_khz,
+                "levels": s.levels}
 
     def trace(self):
         """Per-item (start, end, unit, task, op) of the last persistent run."""
@@ -213,7 +223,7 @@ def info_value(cell) -> int:
     return 0 if v == 2 ** 31 - 1 else v
 
 
-def factorize(kind, A, n, t, mode="subgraph", prec=PREC_FP64, lda=None, **exec_kw):
+def factorize(kind, A, n, t, mode="tiled", prec=PREC_FP64, lda=None, **exec_kw):
     """Tiled LU ('lu') or Cholesky ('chol') of the column-major device matrix A
     in place.  Returns (info, stats)."""
     import torch

@@ -33,7 +33,8 @@ class ExecStats(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("grid", ctypes.c_int), ("units", ctypes.c_longlong),
                 ("items", ctypes.c_longlong), ("edges", ctypes.c_longlong),
                 ("flops", ctypes.c_double), ("critical_path_us", ctypes.c_double),
-                ("priority_units", ctypes.c_int * 4), ("wall_clock_khz", ctypes.c_double)]
+                ("priority_units", ctypes.c_int * 4), ("wall_clock_khz", ctypes.c_double),
+                ("levels", ctypes.c_int)]
 
 
 _vp, _i, _d, _c = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char
@@ -77,9 +78,12 @@ SIGNATURES = {
     "mu_dag_task_deps": (_i, [_vp, _i, _ip, _i, _ip]),
     "mu_dag_add_edge": (_i, [_vp, _i, _i]),
     "mu_dag_set_tag": (_i, [_vp, _i]),
+    "mu_dag_task_info": (_i, [_vp, _i, _ip, _ip, _ip, _i, _ip]),
     "mu_exec_create": (_i, [_vp, _i, ctypes.POINTER(ExecOpts), ctypes.POINTER(_vp)]),
     "mu_exec_launch": (_i, [_vp, _vp]),
     "mu_exec_check": (_i, [_vp]),
+    "mu_exec_launch_level": (_i, [_vp, _i, _vp]),
+    "mu_exec_level_items": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong)]),
     "mu_exec_stats": (_i, [_vp, ctypes.POINTER(ExecStats)]),
     "mu_exec_trace": (_i, [_vp, ctypes.POINTER(ctypes.c_ulonglong), _sz]),
     "mu_exec_item_units": (_i, [_vp, _ip, _ip, _ip]),
@@ -92,6 +96,9 @@ SIGNATURES = {
     "mu_verify_lu_l1_host": (_d, [_vp, _vp, _i]),
     "mu_verify_chol_l1_host": (_d, [_vp, _vp, _i]),
     "mu_scaled_residual": (_i, [_vp, _vp, _i, _i, _i, _vp, ctypes.POINTER(_d), _vp]),
+    "mu_residual_norms": (_i, [_vp, _vp, _i, _i, _i, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d),
+                               ctypes.POINTER(_d), _vp]),
+    "mu_dag_emulate": (_i, [_vp, _i, _d, _i, ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 _LIB = None

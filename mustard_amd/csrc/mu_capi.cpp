@@ -17,6 +17,19 @@
 #include "mu_dag.hpp"
 #include "mu_runtime.hpp"
 
+namespace mu {
+struct EmuResult {
+    uint64_t executed = 0;
+    uint64_t duplicates = 0;
+    uint64_t order_violations = 0;
+    uint32_t err = 0;
+    uint32_t done = 0;
+    uint32_t nitems = 0;
+};
+bool emulate(const Dag &dag, const LowerOpts &lo, int threads, double timeout_s, int bump_unit,
+             EmuResult &res, std::string &err);
+}  // namespace mu
+
 struct mu_dag_s {
     mu::Dag dag;
 };
@@ -444,6 +457,23 @@ int mu_dag_add_edge(mu_dag_t dag, int src, int dst)
     return MU_OK;
 }
 
+int mu_dag_task_info(mu_dag_t dag, int task, int *w_row, int *w_col, int *ops, int max_ops,
+                     int *nops)
+{
+    if (!dag || task < 0 || task >= (int)dag->dag.tasks.size())
+        return fail(MU_EINVAL, "mu_dag_task_info: bad task");
+    const mu::TaskRec &t = dag->dag.tasks[task];
+    if (w_row)
+        *w_row = t.w.first;
+    if (w_col)
+        *w_col = t.w.second;
+    if (nops)
+        *nops = (int)t.ops.size();
+    for (int i = 0; ops && i < (int)t.ops.size() && i < max_ops; i++)
+        ops[i] = t.ops[i].op;
+    return MU_OK;
+}
+
 int mu_dag_set_tag(mu_dag_t dag, int tag)
 {
     if (!dag)
@@ -459,6 +489,11 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
         return fail(MU_EINVAL, "mu_exec_create: bad arguments");
     if (dag->dag.current() >= 0)
         return fail(MU_ESTATE, "mu_exec_create: a task is still being recorded");
+    if (mode == MU_MODE_SUBGRAPH)
+        return fail(MU_ENOTSUP,
+                    "the persistent device scheduler (--subgraph) is switched off in this "
+                    "build: its first runs on the MI355X did not terminate, and the cause is "
+                    "not yet found; use --tiled (level-batched hipGraph)");
     mu::LowerOpts lo;
     mu::ExecOpts eo;
     if (opts) {
@@ -496,6 +531,24 @@ int mu_exec_launch(mu_exec_t exec, void *stream)
     return MU_OK;
 }
 
+int mu_exec_launch_level(mu_exec_t exec, int level, void *stream)
+{
+    if (!exec)
+        return fail(MU_EINVAL, "exec is NULL");
+    std::string err;
+    if (mu::exec_launch_level(exec->e, level, (hipStream_t)stream, err) != 0)
+        return fail(MU_EINVAL, err);
+    return MU_OK;
+}
+
+int mu_exec_level_items(mu_exec_t exec, int level, long long *items)
+{
+    if (!exec || !items || level < 0 || level >= exec->e->levels)
+        return fail(MU_EINVAL, "mu_exec_level_items: bad arguments");
+    *items = (long long)(exec->e->level_off[level + 1] - exec->e->level_off[level]);
+    return MU_OK;
+}
+
 int mu_exec_check(mu_exec_t exec)
 {
     if (!exec)
@@ -520,6 +573,7 @@ int mu_exec_stats(mu_exec_t exec, mu_exec_stats_t *s)
         s->edges += (long long)v.size();
     s->flops = e->low.flops;
     s->critical_path_us = e->low.critical_path;
+    s->levels = e->levels;
     for (const auto &u : e->low.units)
         s->priority_units[u.prio & 3]++;
     s->wall_clock_khz = (double)e->wall_khz;
@@ -564,6 +618,24 @@ int mu_exec_destroy(mu_exec_t exec)
         return MU_OK;
     mu::exec_destroy(exec->e);
     delete exec;
+    return MU_OK;
+}
+
+int mu_dag_emulate(mu_dag_t dag, int threads, double timeout_s, int bump_unit,
+                   long long *results)
+{
+    if (!dag || !results || threads < 1 || timeout_s <= 0)
+        return fail(MU_EINVAL, "mu_dag_emulate: bad arguments");
+    mu::EmuResult r;
+    std::string err;
+    if (!mu::emulate(dag->dag, mu::LowerOpts(), threads, timeout_s, bump_unit, r, err))
+        return fail(MU_EINVAL, err);
+    results[0] = (long long)r.executed;
+    results[1] = (long long)r.duplicates;
+    results[2] = (long long)r.order_violations;
+    results[3] = (long long)r.err;
+    results[4] = (long long)r.done;
+    results[5] = (long long)r.nitems;
     return MU_OK;
 }
 
@@ -777,17 +849,17 @@ double mu_verify_chol_l1_host(const double *A, const double *F, int n)
     return e;
 }
 
-int mu_scaled_residual(const double *A, const double *F, int n, int lda, int chol, double *work,
-                       double *result, void *stream)
+int mu_residual_norms(const double *A, const double *F, int n, int lda, int chol, double *work,
+                      double *frob_res, double *frob_a, double *l1_res, void *stream)
 {
-    if (!A || !F || !work || !result || n <= 0 || lda < n)
-        return fail(MU_EINVAL, "mu_scaled_residual: bad arguments");
+    if (!A || !F || !work || n <= 0 || lda < n)
+        return fail(MU_EINVAL, "mu_residual_norms: bad arguments");
     hipStream_t s = (hipStream_t)stream;
     const size_t nn = (size_t)n * n;
     double *L = work, *U = work + nn, *R = work + 2 * nn;
     double *acc = nullptr;
-    HIPTRY(hipMalloc(&acc, 2 * sizeof(double)));
-    HIPTRY(hipMemsetAsync(acc, 0, 2 * sizeof(double), s));
+    HIPTRY(hipMalloc(&acc, 3 * sizeof(double)));
+    HIPTRY(hipMemsetAsync(acc, 0, 3 * sizeof(double), s));
     int rc = 0;
     if (mu::split_lu(F, n, lda, L, U, n, chol, s) || mu::copy_matrix(A, lda, R, n, n, n, s))
         rc = fail(MU_EHIP, "kernel launch failed");
@@ -795,9 +867,10 @@ int mu_scaled_residual(const double *A, const double *F, int n, int lda, int cho
         rc = chol ? mu_dgemm(stream, 'N', 'T', n, n, n, -1.0, L, n, L, n, 1.0, R, n)
                   : mu_dgemm(stream, 'N', 'N', n, n, n, -1.0, L, n, U, n, 1.0, R, n);
     if (!rc && (mu::sub_frob2(R, n, nullptr, 0, n, n, acc, s) ||
-                mu::sub_frob2(A, lda, nullptr, 0, n, n, acc + 1, s)))
+                mu::sub_frob2(A, lda, nullptr, 0, n, n, acc + 1, s) ||
+                mu::sum_abs(R, n, n, n, acc + 2, s)))
         rc = fail(MU_EHIP, "kernel launch failed");
-    double h[2] = {0, 0};
+    double h[3] = {0, 0, 0};
     if (!rc) {
         hipError_t e = hipMemcpyAsync(h, acc, sizeof h, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess)
@@ -806,8 +879,26 @@ int mu_scaled_residual(const double *A, const double *F, int n, int lda, int cho
             rc = fail(MU_EHIP, hipGetErrorString(e));
     }
     (void)hipFree(acc);
+    if (!rc) {
+        if (frob_res)
+            *frob_res = std::sqrt(h[0]);
+        if (frob_a)
+            *frob_a = std::sqrt(h[1]);
+        if (l1_res)
+            *l1_res = h[2];
+    }
+    return rc;
+}
+
+int mu_scaled_residual(const double *A, const double *F, int n, int lda, int chol, double *work,
+                       double *result, void *stream)
+{
+    if (!result)
+        return fail(MU_EINVAL, "result is NULL");
+    double fr = 0, fa = 0;
+    int rc = mu_residual_norms(A, F, n, lda, chol, work, &fr, &fa, nullptr, stream);
     if (!rc)
-        *result = std::sqrt(h[0]) / (std::sqrt(h[1]) * n * DBL_EPSILON);
+        *result = fr / (fa * n * DBL_EPSILON);
     return rc;
 }
 

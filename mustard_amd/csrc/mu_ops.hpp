@@ -18,6 +18,17 @@
 namespace mu {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+// Operand pointers come from descriptors in memory, so the compiler cannot
+// infer their address space; they always point to device global memory, and
+// saying so turns flat accesses into global (vector memory) instructions.
+#define MU_G __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ MU_G T *gp(T *p)
+{
+    return (MU_G T *)p;
+}
 
 // v_mfma_f64_16x16x4_f64.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // it receives D[4*e + l/16][l%16] in element e (register e holds rows 4e..4e+3).
@@ -43,18 +54,18 @@ constexpr int LDS_K = GEMM_BM * PK;   // doubles per k-contiguous buffer
 // r < rows, k < kcols.  Thread t covers rows 2(t%64), 2(t%64)+1 of columns
 // t/64 + 4q, q = 0..3.
 template <bool VEC>
-__device__ __forceinline__ void ld_icontig(double (&v)[8], const double *p, int ld, int rows,
+__device__ __forceinline__ void ld_icontig(double (&v)[8], const MU_G double *p, int ld, int rows,
                                            int kcols)
 {
     const int t = threadIdx.x, r = (t & 63) * 2, c0 = t >> 6;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int c = c0 + 4 * q;
-        const double *s = p + r + (size_t)c * ld;
+        const MU_G double *s = p + r + (size_t)c * ld;
         if (VEC && c < kcols && r + 1 < rows) {
-            double2 x = *reinterpret_cast<const double2 *>(s);
-            v[2 * q] = x.x;
-            v[2 * q + 1] = x.y;
+            const v2d x = *reinterpret_cast<const MU_G v2d *>(s);
+            v[2 * q] = x[0];
+            v[2 * q + 1] = x[1];
         } else {
             v[2 * q] = (c < kcols && r < rows) ? s[0] : 0.0;
             v[2 * q + 1] = (c < kcols && r + 1 < rows) ? s[1] : 0.0;
@@ -67,28 +78,28 @@ __device__ __forceinline__ void st_icontig(double *S, const double (&v)[8])
     const int t = threadIdx.x, r = (t & 63) * 2, c0 = t >> 6;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        double2 x;
-        x.x = v[2 * q];
-        x.y = v[2 * q + 1];
-        *reinterpret_cast<double2 *>(S + (c0 + 4 * q) * PI + r) = x;
+        v2d x;
+        x[0] = v[2 * q];
+        x[1] = v[2 * q + 1];
+        *reinterpret_cast<v2d *>(S + (c0 + 4 * q) * PI + r) = x;
     }
 }
 
 // Global -> registers, k-contiguous operand: element (r, k) at p[k + r*ld].
 // Thread t covers k = 2(t%8), 2(t%8)+1 of rows t/8 + 32q, q = 0..3.
 template <bool VEC>
-__device__ __forceinline__ void ld_kcontig(double (&v)[8], const double *p, int ld, int rows,
+__device__ __forceinline__ void ld_kcontig(double (&v)[8], const MU_G double *p, int ld, int rows,
                                            int kcols)
 {
     const int t = threadIdx.x, k = (t & 7) * 2, r0 = t >> 3;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int r = r0 + 32 * q;
-        const double *s = p + k + (size_t)r * ld;
+        const MU_G double *s = p + k + (size_t)r * ld;
         if (VEC && r < rows && k + 1 < kcols) {
-            double2 x = *reinterpret_cast<const double2 *>(s);
-            v[2 * q] = x.x;
-            v[2 * q + 1] = x.y;
+            const v2d x = *reinterpret_cast<const MU_G v2d *>(s);
+            v[2 * q] = x[0];
+            v[2 * q + 1] = x[1];
         } else {
             v[2 * q] = (r < rows && k < kcols) ? s[0] : 0.0;
             v[2 * q + 1] = (r < rows && k + 1 < kcols) ? s[1] : 0.0;
@@ -121,7 +132,7 @@ __device__ __forceinline__ double frag_b(const double *S, int k, int j)
 }
 
 template <bool TA, bool TB, bool VEC>
-__device__ void gemm_block(const UnitDesc &u, int bi, int bj, bool lower_only, double *smem)
+__device__ __attribute__((always_inline)) inline void gemm_block(const UnitDesc &u, int bi, int bj, bool lower_only, double *smem)
 {
     constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = GEMM_BK;
     constexpr int SA = TA ? LDS_K : LDS_I;
@@ -132,8 +143,8 @@ __device__ void gemm_block(const UnitDesc &u, int bi, int bj, bool lower_only, d
     const int rows = min(BM, u.m - i0), cols = min(BN, u.n - j0);
     const int K = u.k;
     // op(A)(i, k): TA ? a[k + i*lda] : a[i + k*lda];  op(B)(k, j): TB ? b[j + k*ldb] : b[k + j*ldb]
-    const double *A = TA ? u.a + (size_t)i0 * u.lda : u.a + i0;
-    const double *B = TB ? u.b + j0 : u.b + (size_t)j0 * u.ldb;
+    const MU_G double *A = TA ? gp(u.a) + (size_t)i0 * u.lda : gp(u.a) + i0;
+    const MU_G double *B = TB ? gp(u.b) + j0 : gp(u.b) + (size_t)j0 * u.ldb;
     double *As0 = smem, *As1 = smem + SA, *Bs0 = smem + 2 * SA, *Bs1 = smem + 2 * SA + SB;
 
     v4d acc[4][4];
@@ -210,7 +221,7 @@ __device__ void gemm_block(const UnitDesc &u, int bi, int bj, bool lower_only, d
                 const int r = wm * 64 + x * 16 + 4 * e + lk;
                 const int c = wn * 64 + y * 16 + li;
                 if (r < rows && c < cols && (!lower_only || i0 + r >= j0 + c)) {
-                    double *p = u.c + (i0 + r) + (size_t)(j0 + c) * u.ldc;
+                    MU_G double *p = gp(u.c) + (i0 + r) + (size_t)(j0 + c) * u.ldc;
                     const double old = beta == 0.0 ? 0.0 : beta * *p;
                     *p = old + alpha * acc[x][y][e];
                 }
@@ -229,16 +240,16 @@ __device__ void gemm_block(const UnitDesc &u, int bi, int bj, bool lower_only, d
 // thread per right-hand side and updates the rows below with FP64 MFMA.
 // ---------------------------------------------------------------------------
 template <bool TT, bool ZR, bool UNIT, bool REV>
-__device__ void trsm_panel(const UnitDesc &u, int panel, double *smem)
+__device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc &u, int panel, double *smem)
 {
     const int n = u.m, p0 = panel * TRSM_W, w = min(TRSM_W, u.n - p0);
     const int np = (n + 15) & ~15;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double *Y = smem;                   // [np][16]
     double *Td = smem + TRSM_NMAX * 16; // [16][17]
-    const double *t = u.a;
+    const MU_G double *t = gp(u.a);
     const size_t ldt = u.lda, ldz = u.ldc;
-    double *z = u.c;
+    MU_G double *z = gp(u.c);
     const double alpha = u.alpha;
     auto tv = [&](int r, int c) -> double {
         if (REV) {
@@ -247,7 +258,7 @@ __device__ void trsm_panel(const UnitDesc &u, int panel, double *smem)
         }
         return TT ? t[c + r * ldt] : t[r + c * ldt];
     };
-    auto zp = [&](int r, int c) -> double * {
+    auto zp = [&](int r, int c) -> MU_G double * {
         if (REV)
             r = n - 1 - r;
         return ZR ? z + (p0 + c) + r * ldz : z + r + (p0 + c) * ldz;
@@ -356,11 +367,11 @@ __device__ __forceinline__ void report_info(const UnitDesc &u, int col)
                                __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ void getrf_small(const UnitDesc &u, double *smem)
+__device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc &u, double *smem)
 {
     const int n = u.m, tid = threadIdx.x;
     double *S = smem;
-    double *a = u.c;
+    MU_G double *a = gp(u.c);
     const size_t lda = u.ldc;
     for (int idx = tid; idx < n * n; idx += WG_THREADS) {
         const int r = idx % n, c = idx / n;
@@ -387,11 +398,11 @@ __device__ void getrf_small(const UnitDesc &u, double *smem)
     }
 }
 
-__device__ void potrf_small(const UnitDesc &u, double *smem)
+__device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc &u, double *smem)
 {
     const int n = u.m, tid = threadIdx.x;
     double *S = smem;
-    double *a = u.c;
+    MU_G double *a = gp(u.c);
     const size_t lda = u.ldc;
     for (int idx = tid; idx < n * n; idx += WG_THREADS) {
         const int r = idx % n, c = idx / n;
@@ -452,8 +463,10 @@ __device__ __forceinline__ void gemm_dispatch_vec(const UnitDesc &u, int bi, int
         gemm_block<TA, TB, false>(u, bi, bj, lo, smem);
 }
 
-// Execute local item `it` of unit `u`.
-__device__ void run_item(const UnitDesc &u, int it, double *smem)
+// Execute local item `it` of unit `u`.  Always inlined, so that LDS stays
+// addressed as LDS and no call stack is needed in the persistent kernel.
+__device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u, int it,
+                                                               double *smem)
 {
     switch (u.op) {
     case OP_GEMM: {

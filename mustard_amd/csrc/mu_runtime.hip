@@ -13,6 +13,7 @@
 
 #include "mu_ops.hpp"
 #include "mu_runtime.hpp"
+#include "mu_sched.hpp"
 
 namespace mu {
 
@@ -37,77 +38,63 @@ __global__ __launch_bounds__(WG_THREADS) void k_unit(UnitDesc d)
 }
 
 // ---------------------------------------------------------------------------
-// persistent scheduler
+// level-batched kernel (graph mode): one launch runs every item of every unit
+// at one level of the unit DAG; block b runs items[b]
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p)
+__global__ __launch_bounds__(WG_THREADS) void k_batch(const UnitDesc *units,
+                                                      const uint32_t *item_unit,
+                                                      const uint32_t *items)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT);
+    __shared__ double smem[LDS_DOUBLES];
+    const uint32_t item = items[blockIdx.x];
+    const uint32_t u = item_unit[item];
+    const UnitDesc *d = units + u;
+    run_item(*d, (int)(item - d->item_base), smem);
 }
 
-// Queue all items of unit u on its priority ring.  The ring never wraps: it
-// holds every item of its priority once per run.
-__device__ void push_unit(const ExecView &ev, uint32_t u)
-{
-    const UnitDesc *d = ev.units + u;
-    const int p = d->prio;
-    const uint32_t cnt = (uint32_t)d->nitems, base = d->item_base;
-    const uint32_t pos = __hip_atomic_fetch_add(&ev.st->tail[p], cnt, __ATOMIC_RELAXED, AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    for (uint32_t r = 0; r < cnt; r++)
-        __hip_atomic_store(ev.slots[p] + pos + r, base + r + 1, __ATOMIC_RELAXED, AGENT);
-}
-
-// One attempt on ring p: an item id, -1 when the ring is empty.
-__device__ int try_pop(const ExecView &ev, int p, uint64_t deadline)
-{
-    uint32_t *head = &ev.st->head[p];
-    for (;;) {
-        uint32_t h = ld_relaxed(head);
-        const uint32_t t = ld_relaxed(&ev.st->tail[p]);
-        if (h >= t)
-            return -1;
-        if (__hip_atomic_compare_exchange_strong(head, &h, h + 1, __ATOMIC_RELAXED,
-                                                 __ATOMIC_RELAXED, AGENT)) {
-            // slot h is reserved by a producer that is writing it now
-            uint32_t v;
-            while ((v = __hip_atomic_load(ev.slots[p] + h, __ATOMIC_ACQUIRE, AGENT)) == 0) {
-                if (wall_clock64() > deadline || ld_relaxed(&ev.st->abort))
-                    return -1;
-                __builtin_amdgcn_s_sleep(1);
-            }
-            return (int)(v - 1);
-        }
+// ---------------------------------------------------------------------------
+// persistent scheduler (protocol in mu_sched.hpp)
+// ---------------------------------------------------------------------------
+struct DevAtom {
+    static __device__ __forceinline__ uint32_t load(uint32_t *p)
+    {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, AGENT);
     }
-}
-
-// Next item for this workgroup, or -1 when the run is over (or aborted).
-__device__ int pop_any(const ExecView &ev)
-{
-    uint64_t t0 = wall_clock64();
-    uint32_t seen = ld_relaxed(&ev.st->done);
-    for (;;) {
-        if (ld_relaxed(&ev.st->abort))
-            return -1;
-        for (int p = 0; p < NPRIO; p++) {
-            const int r = try_pop(ev, p, t0 + ev.timeout_ticks);
-            if (r >= 0)
-                return r;
-        }
-        const uint32_t d = ld_relaxed(&ev.st->done);
-        if (d >= ev.nitems)
-            return -1;
-        const uint64_t now = wall_clock64();
-        if (d != seen) {
-            seen = d;
-            t0 = now;
-        } else if (now - t0 > ev.timeout_ticks) {
-            __hip_atomic_store(&ev.st->err, 1u, __ATOMIC_RELAXED, AGENT);
-            __hip_atomic_store(&ev.st->abort, 1u, __ATOMIC_RELAXED, AGENT);
-            return -1;
-        }
-        __builtin_amdgcn_s_sleep(2);
+    static __device__ __forceinline__ uint32_t load_acquire(uint32_t *p)
+    {
+        return __hip_atomic_load(p, __ATOMIC_ACQUIRE, AGENT);
     }
-}
+    static __device__ __forceinline__ void store(uint32_t *p, uint32_t v)
+    {
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT);
+    }
+    static __device__ __forceinline__ uint32_t add(uint32_t *p, uint32_t v)
+    {
+        return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, AGENT);
+    }
+    static __device__ __forceinline__ uint32_t add_release(uint32_t *p, uint32_t v)
+    {
+        return __hip_atomic_fetch_add(p, v, __ATOMIC_RELEASE, AGENT);
+    }
+    static __device__ __forceinline__ int32_t add_acq_rel(int32_t *p, int32_t v)
+    {
+        return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, AGENT);
+    }
+    // compare-and-swap; returns the value found (== expected on success)
+    static __device__ __forceinline__ uint32_t cas(uint32_t *p, uint32_t expected,
+                                                   uint32_t desired)
+    {
+        __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, AGENT);
+        return expected;
+    }
+    static __device__ __forceinline__ void fence_release()
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    }
+    static __device__ __forceinline__ uint64_t now() { return (uint64_t)wall_clock64(); }
+    static __device__ __forceinline__ void pause() { __builtin_amdgcn_s_sleep(2); }
+};
 
 __global__ __launch_bounds__(WG_THREADS) void k_persistent(ExecView ev)
 {
@@ -116,7 +103,7 @@ __global__ __launch_bounds__(WG_THREADS) void k_persistent(ExecView ev)
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) {
-            const int it = pop_any(ev);
+            const int it = sched_pop<DevAtom>(ev);
             s_item = it;
             s_unit = it >= 0 ? (int)ev.item_unit[it] : -1;
         }
@@ -136,20 +123,14 @@ __global__ __launch_bounds__(WG_THREADS) void k_persistent(ExecView ev)
         if (tid == 0) {
             if (ev.trace)
                 ev.trace[2 * (size_t)item + 1] = wall_clock64();
-            s_last = __hip_atomic_fetch_add(ev.items_left + u, -1, __ATOMIC_ACQ_REL, AGENT) == 1;
+            s_last = sched_item_done<DevAtom>(ev, (uint32_t)u);
         }
         __syncthreads();
-        if (s_last) {
-            const uint32_t b = d->succ_begin, e = d->succ_end;
-            for (uint32_t s = b + tid; s < e; s += WG_THREADS) {
-                const uint32_t v = ev.succ[s];
-                if (__hip_atomic_fetch_add(ev.dep_count + v, -1, __ATOMIC_ACQ_REL, AGENT) == 1)
-                    push_unit(ev, v);
-            }
-        }
+        if (s_last)
+            sched_release<DevAtom>(ev, (uint32_t)u, (uint32_t)tid, WG_THREADS);
         __syncthreads();
         if (tid == 0)
-            __hip_atomic_fetch_add(&ev.st->done, 1u, __ATOMIC_RELEASE, AGENT);
+            sched_count<DevAtom>(ev);
     }
 }
 
@@ -225,27 +206,74 @@ int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts
             item_unit[L.units[u].item_base + i] = u;
 
     if (mode == MODE_GRAPH) {
-        e->preds.assign(nu, {});
+        // Level of each unit = longest chain of units before it.  Units of one
+        // level are independent, so one kernel runs all their items; the
+        // levels form a chain of kernel nodes in one hipGraph.
+        std::vector<int> level(nu, 0);
+        int nlev = nu ? 1 : 0;
         for (uint32_t u = 0; u < nu; u++)
-            for (uint32_t v : L.succ[u])
-                e->preds[v].push_back(u);
+            for (uint32_t v : L.succ[u]) {
+                level[v] = std::max(level[v], level[u] + 1);
+                nlev = std::max(nlev, level[v] + 1);
+            }
+        std::vector<std::vector<uint32_t>> lv(nlev);
+        for (uint32_t u = 0; u < nu; u++)
+            for (int i = 0; i < L.units[u].nitems; i++)
+                lv[level[u]].push_back(L.units[u].item_base + i);
+        std::vector<uint32_t> items;
+        std::vector<size_t> off;
+        for (auto &v : lv) {
+            off.push_back(items.size());
+            items.insert(items.end(), v.begin(), v.end());
+        }
+        size_t o = 0;
+        auto take = [&](size_t bytes) {
+            size_t r = align_up(o, 256);
+            o = r + bytes;
+            return r;
+        };
+        const size_t o_units = take(sizeof(UnitDesc) * std::max<uint32_t>(nu, 1));
+        const size_t o_iu = take(sizeof(uint32_t) * std::max<size_t>(L.nitems, 1));
+        const size_t o_items = take(sizeof(uint32_t) * std::max<size_t>(items.size(), 1));
+        e->dbytes = o;
+        HIPCHK(hipMalloc(&e->dbuf, e->dbytes));
+        if (nu)
+            HIPCHK(hipMemcpy(e->dbuf + o_units, L.units.data(), sizeof(UnitDesc) * nu,
+                             hipMemcpyHostToDevice));
+        if (L.nitems)
+            HIPCHK(hipMemcpy(e->dbuf + o_iu, item_unit.data(), sizeof(uint32_t) * L.nitems,
+                             hipMemcpyHostToDevice));
+        if (!items.empty())
+            HIPCHK(hipMemcpy(e->dbuf + o_items, items.data(), sizeof(uint32_t) * items.size(),
+                             hipMemcpyHostToDevice));
+        const UnitDesc *du = reinterpret_cast<const UnitDesc *>(e->dbuf + o_units);
+        const uint32_t *diu = reinterpret_cast<const uint32_t *>(e->dbuf + o_iu);
+        const uint32_t *ditems = reinterpret_cast<const uint32_t *>(e->dbuf + o_items);
         HIPCHK(hipGraphCreate(&e->graph, 0));
-        std::vector<hipGraphNode_t> node(nu, nullptr);
-        for (uint32_t u = 0; u < nu; u++) {
-            UnitDesc d = L.units[u];
-            void *args[] = {&d};
+        hipGraphNode_t prev = nullptr;
+        for (int l = 0; l < nlev; l++) {
+            const uint32_t *first = ditems + off[l];
+            void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
             hipKernelNodeParams kp;
             std::memset(&kp, 0, sizeof kp);
-            kp.func = reinterpret_cast<void *>(k_unit);
-            kp.gridDim = dim3(d.nitems);
+            kp.func = reinterpret_cast<void *>(k_batch);
+            kp.gridDim = dim3((unsigned)lv[l].size());
             kp.blockDim = dim3(WG_THREADS);
             kp.kernelParams = args;
-            std::vector<hipGraphNode_t> deps;
-            for (uint32_t p : e->preds[u])
-                deps.push_back(node[p]);
-            HIPCHK(hipGraphAddKernelNode(&node[u], e->graph, deps.data(), deps.size(), &kp));
+            hipGraphNode_t node;
+            HIPCHK(hipGraphAddKernelNode(&node, e->graph, prev ? &prev : nullptr, prev ? 1 : 0,
+                                         &kp));
+            prev = node;
         }
-        HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+        e->levels = nlev;
+        e->level_items = std::move(items);
+        e->level_off.assign(off.begin(), off.end());
+        e->level_off.push_back(e->level_items.size());
+        e->level_args[0] = (const void *)du;
+        e->level_args[1] = (const void *)diu;
+        e->level_args[2] = (const void *)ditems;
+        if (nlev)
+            HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
         *out = e;
         return 0;
     }
@@ -318,8 +346,12 @@ int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts
     ev.trace = e->trace;
     int khz = 0;
     HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device));
-    e->wall_khz = khz > 0 ? (uint64_t)khz : 100000;
+    // the constant wall clock runs at 1 MHz .. 1 GHz; anything else is a
+    // misreported rate, and 100 MHz (the CDNA value) is used instead
+    e->wall_khz = (khz >= 1000 && khz <= 1000000) ? (uint64_t)khz : 100000;
     ev.timeout_ticks = (uint64_t)(eopts.timeout_s * 1000.0 * (double)e->wall_khz);
+    // backstop independent of the clock: one empty poll takes well over 50 ns
+    ev.poll_limit = (uint64_t)(eopts.timeout_s * 2.0e7) + 1000;
 
     int ncu = 0, per_cu = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, e->device));
@@ -336,7 +368,8 @@ int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts
 int exec_launch(Exec *e, hipStream_t s, std::string &err)
 {
     if (e->mode == MODE_GRAPH) {
-        HIPCHK(hipGraphLaunch(e->gexec, s));
+        if (e->gexec)
+            HIPCHK(hipGraphLaunch(e->gexec, s));
         return 0;
     }
     if (e->mode == MODE_STREAM)
@@ -353,6 +386,21 @@ int exec_launch(Exec *e, hipStream_t s, std::string &err)
     if (e->ev.nitems == 0)
         return 0;
     hipLaunchKernelGGL(k_persistent, dim3(e->grid), dim3(WG_THREADS), 0, s, e->ev);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int exec_launch_level(Exec *e, int level, hipStream_t s, std::string &err)
+{
+    if (e->mode != MODE_GRAPH || level < 0 || level >= e->levels) {
+        err = "exec_launch_level: tiled executables only, level out of range";
+        return -1;
+    }
+    const UnitDesc *du = (const UnitDesc *)e->level_args[0];
+    const uint32_t *diu = (const uint32_t *)e->level_args[1];
+    const uint32_t *first = (const uint32_t *)e->level_args[2] + e->level_off[level];
+    const unsigned n = (unsigned)(e->level_off[level + 1] - e->level_off[level]);
+    hipLaunchKernelGGL(k_batch, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -524,6 +572,30 @@ int sub_frob2(const double *A, int lda, const double *B, int ldb, int m, int n, 
 {
     hipLaunchKernelGGL(k_sub_frob2, dim3(2048), dim3(256), 0, s, A, (size_t)lda, B, (size_t)ldb,
                        m, n, acc);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_sum_abs(const double *A, size_t lda, int m, int n, double *acc)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)m * n;
+         k += (size_t)gridDim.x * blockDim.x)
+        s += fabs(A[k % m + (k / m) * lda]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicAdd(acc, red[0]);
+}
+
+int sum_abs(const double *A, int lda, int m, int n, double *acc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum_abs, dim3(2048), dim3(256), 0, s, A, (size_t)lda, m, n, acc);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

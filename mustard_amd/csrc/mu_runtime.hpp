@@ -10,7 +10,7 @@
 namespace mu {
 
 enum Mode : int {
-    MODE_GRAPH = 0,       // one hipGraph kernel node per unit, explicit edges ("--tiled")
+    MODE_GRAPH = 0,       // one hipGraph, one batched kernel node per DAG level ("--tiled")
     MODE_PERSISTENT = 1,  // one persistent scheduler kernel ("--subgraph")
     MODE_STREAM = 2,      // units launched in topological order on the stream
 };
@@ -38,12 +38,17 @@ struct Exec {
     // graph mode
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    std::vector<std::vector<uint32_t>> preds;
+    int levels = 0;
+    std::vector<uint32_t> level_items;
+    std::vector<size_t> level_off;
+    const void *level_args[3] = {nullptr, nullptr, nullptr};
 };
 
 int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts &eopts,
                 Exec **out, std::string &err);
 int exec_launch(Exec *e, hipStream_t s, std::string &err);
+// Tiled mode: launch the kernel of one DAG level alone (for per-launch timing).
+int exec_launch_level(Exec *e, int level, hipStream_t s, std::string &err);
 // After the stream has finished: scheduler status (0 = fine).
 int exec_check(Exec *e, std::string &err);
 int exec_trace(Exec *e, std::vector<uint64_t> &stamps, std::string &err);
@@ -62,6 +67,7 @@ int split_lu(const double *F, int n, int ldf, double *L, double *U, int ld, int 
              hipStream_t s);
 int sub_frob2(const double *A, int lda, const double *B, int ldb, int m, int n, double *acc,
               hipStream_t s);
+int sum_abs(const double *A, int lda, int m, int n, double *acc, hipStream_t s);
 int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, hipStream_t s);
 
 }  // namespace mu

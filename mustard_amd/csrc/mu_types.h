@@ -91,6 +91,7 @@ struct ExecView {
     uint32_t nunits;
     uint32_t nitems;
     uint64_t timeout_ticks;        // wall-clock ticks without progress before abort
+    uint64_t poll_limit;           // empty polls without progress before abort
     uint64_t *trace;               // optional: per item (start, end) wall-clock stamps
 };
 

@@ -1,0 +1,138 @@
+"""CPU tests of libmustard_hip.so: exports, DAG recording, the scheduler
+protocol (host emulation) and error paths.  No GPU is touched."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import mustard_amd as mu
+from mustard_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_exports_every_declared_symbol():
+    with open(os.path.join(ROOT, "include", "mustard_hip.h")) as f:
+        decl = set(re.findall(r"\b(mu_[a-z0-9_]+)\s*\(", f.read()))
+    L = mu.lib()
+    missing = [s for s in decl if not hasattr(L, s)]
+    assert not missing, missing
+    # the Python binding covers every declared entry point
+    assert decl <= set(_lib.SIGNATURES), decl - set(_lib.SIGNATURES)
+
+
+def _fake(addr=1 << 20):
+    return ctypes.c_void_p(addr)
+
+
+@pytest.mark.parametrize("kind", ["lu", "chol"])
+@pytest.mark.parametrize("T", [1, 2, 3, 4, 5, 6, 9])
+def test_recorded_deps_equal_oracle(oracle, kind, T):
+    d = mu.Dag()
+    if kind == "lu":
+        d.record_lu(_fake(), 60 * T, T)
+    else:
+        d.record_cholesky(_fake(), 60 * T, T)
+    ops, deps = oracle.task_dag(kind, T)
+    assert d.num_tasks() == len(ops)
+    assert [d.deps(t) for t in range(len(ops))] == deps
+    names = {1: "GEMM", 2: "SYRK", 3: "TRSM", 4: "GETRF", 5: "POTRF"}
+    for t, (op, w, _) in enumerate(ops):
+        r, c, o, n = ctypes.c_int(), ctypes.c_int(), (ctypes.c_int * 4)(), ctypes.c_int()
+        mu.call("mu_dag_task_info", d.h, t, ctypes.byref(r), ctypes.byref(c), o, 4,
+                ctypes.byref(n))
+        assert (r.value, c.value) == w and n.value == 1
+        assert names[o[0]] == op.split("_")[0]
+
+
+def test_recorded_counts_at_24(oracle):
+    for kind, tasks, edges in (("lu", 4900, 13524), ("chol", 2600, 6900)):
+        d = mu.Dag()
+        (d.record_lu if kind == "lu" else d.record_cholesky)(_fake(), 24000, 24)
+        assert d.num_tasks() == tasks
+        assert sum(len(d.deps(t)) for t in range(tasks)) == edges
+
+
+def _emulate(d, threads, timeout=20.0, bump=-1):
+    r = (ctypes.c_longlong * 6)()
+    mu.call("mu_dag_emulate", d.h, threads, timeout, bump, r)
+    return list(r)
+
+
+@pytest.mark.parametrize("n,t,kind", [(600, 2, "lu"), (600, 6, "chol"), (3000, 3, "lu"),
+                                      (24000, 24, "lu"), (24000, 24, "chol"), (4000, 2, "lu")])
+@pytest.mark.parametrize("threads", [1, 7, 48])
+def test_scheduler_protocol_emulated(n, t, kind, threads):
+    """Every item runs exactly once, after all items of its predecessor units,
+    and the run terminates (the device protocol, run by host threads)."""
+    d = mu.Dag()
+    (d.record_lu if kind == "lu" else d.record_cholesky)(_fake(), n, t)
+    executed, dup, viol, err, done, total = _emulate(d, threads)
+    assert (executed, dup, viol, err, done) == (total, 0, 0, 0, total)
+
+
+def test_scheduler_watchdog_detects_deadlock():
+    d = mu.Dag().record_lu(_fake(), 600, 2)
+    executed, dup, viol, err, done, total = _emulate(d, 4, timeout=0.2, bump=7)
+    assert err == 1 and done < total and dup == 0 and viol == 0
+
+
+def test_dag_error_paths():
+    d = mu.Dag()
+    with pytest.raises(mu.MustardError) as e:
+        d.end_task()
+    assert e.value.code == _lib.MU_ESTATE
+    d.begin_task((0, 0), [(0, 0)])
+    with pytest.raises(mu.MustardError) as e:
+        d.begin_task((0, 1), [])
+    assert e.value.code == _lib.MU_ESTATE
+    d.end_task()
+    d.begin_task((1, 1), [(0, 0)])
+    d.end_task()
+    assert d.deps(1) == [0]
+    with pytest.raises(mu.MustardError):
+        d.add_edge(1, 0)  # edges run from earlier to later tasks
+    with pytest.raises(mu.MustardError):
+        mu.call("mu_record_tiled_lu", d.h, _fake(), 600, 7, 600, None, 0)
+
+
+def test_tile_op_argument_errors():
+    s = ctypes.c_void_p(0)
+    L = mu.lib()
+    assert L.mu_dgemm(s, b"X", b"N", 2, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 2) == _lib.MU_EINVAL
+    assert L.mu_dgemm(s, b"N", b"N", 4, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 4) == _lib.MU_EINVAL
+    assert L.mu_dtrsm(s, b"Q", b"L", b"N", b"U", 2, 2, 1.0, None, 2, None, 2) == _lib.MU_EINVAL
+    assert L.mu_dsyrk(s, b"U", b"N", 2, 2, 1.0, None, 2, 0.0, None, 2) == _lib.MU_ENOTSUP
+    assert L.mu_dpotrf(s, b"U", 2, None, 2, None) == _lib.MU_ENOTSUP
+    assert L.mu_dgetrf_nopiv(s, 4, None, 2, None) == _lib.MU_EINVAL
+    assert b"leading" in L.mu_last_error() or len(L.mu_last_error()) > 0
+
+
+def test_capture_records_instead_of_running():
+    """Ops issued on a stream that a DAG is recording are added to its current
+    task (nothing runs, so fake pointers are fine here)."""
+    d = mu.Dag()
+    key = ctypes.c_void_p(12345)
+    d.capture(key)
+    d.begin_task((0, 0), [(0, 0)])
+    mu.call("mu_dgetrf_nopiv", key, 100, _fake(), 100, None)
+    mu.call("mu_dgemm", key, b"N", b"T", 8, 8, 8, -1.0, _fake(), 8, _fake(), 8, 1.0, _fake(), 8)
+    d.end_task()
+    d.end_capture(key)
+    n = ctypes.c_int()
+    ops = (ctypes.c_int * 4)()
+    mu.call("mu_dag_task_info", d.h, 0, None, None, ops, 4, ctypes.byref(n))
+    assert n.value == 2 and list(ops[:2]) == [4, 1]
+    with pytest.raises(mu.MustardError):
+        d.end_capture(key)
+
+
+def test_persistent_mode_is_switched_off():
+    """--subgraph (the persistent scheduler) is refused with a clear error in
+    this build rather than run (see tests/test_gpu_tiled.py)."""
+    d = mu.Dag().record_lu(_fake(), 600, 2)
+    with pytest.raises(mu.MustardError) as e:
+        mu.Exec(d, "subgraph")
+    assert e.value.code == _lib.MU_ENOTSUP
+    assert "switched off" in str(e.value)

@@ -15,7 +15,16 @@ from tests.util import close_rel, spd
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["subgraph", "tiled", "stream"]
+MODES = ["tiled", "stream"]
+
+# The persistent device scheduler did not terminate in its first two runs on
+# the MI355X (LU N=600, 2x2 tiles); its cause is not found yet, so the mode is
+# switched off in the library (mu_exec_create returns MU_ENOTSUP) and its GPU
+# tests are skipped.  The queue/counter protocol itself is tested on the CPU
+# (tests/test_capi.py, mu_dag_emulate).
+PERSISTENT_OFF = pytest.mark.skip(
+    reason="persistent scheduler switched off: first MI355X runs did not terminate, cause "
+           "not found yet (protocol covered by the CPU emulation tests)")
 
 
 def _mu():
@@ -82,7 +91,7 @@ def test_tiled_vs_oracle(torch_gpu, oracle, kind, n, t):
         assert oracle.tiled_lu(ref, t) == 0
     else:
         assert oracle.tiled_cholesky(ref, t) == 0
-    for mode in (["subgraph", "tiled"] if n <= 3000 else ["subgraph"]):
+    for mode in (["tiled", "stream"] if n <= 3000 else ["tiled"]):
         F, info, res, _ = _run(torch_gpu, kind, A, t, mode)
         assert info == 0
         if kind == "lu":
@@ -93,6 +102,7 @@ def test_tiled_vs_oracle(torch_gpu, oracle, kind, n, t):
         assert res <= 10.0, mode
 
 
+@PERSISTENT_OFF
 @pytest.mark.parametrize("grid", [1, 3, 64])
 def test_persistent_small_grids(torch_gpu, oracle, grid):
     """The scheduler is correct with any number of workgroups, down to one."""
@@ -105,6 +115,7 @@ def test_persistent_small_grids(torch_gpu, oracle, grid):
     assert close_rel(F, ref) <= 1.0
 
 
+@PERSISTENT_OFF
 def test_persistent_trace(torch_gpu, oracle):
     mu = _mu()
     n, t = 1200, 4
@@ -130,6 +141,7 @@ def test_persistent_trace(torch_gpu, oracle):
             assert last[p] <= first[task]
 
 
+@PERSISTENT_OFF
 def test_persistent_timeout_aborts_cleanly(torch_gpu, oracle):
     """A zero time-out makes idle workgroups abort; every workgroup exits and
     the run reports MU_ERUN instead of hanging."""
@@ -146,15 +158,14 @@ def test_persistent_timeout_aborts_cleanly(torch_gpu, oracle):
 
 
 def test_repeated_launches_identical(torch_gpu, oracle):
-    """Re-running the same executable (counters and rings reset on the device)
-    gives bit-identical factors."""
+    """Re-running the same executable gives bit-identical factors."""
     mu = _mu()
     n, t = 2000, 4
     A = oracle.gen_ref(n)
     dA = mu.to_device(A)
     orig = dA.clone()
     dag = mu.Dag().record_lu(mu.ptr(dA), n, t)
-    ex = mu.Exec(dag, "subgraph")
+    ex = mu.Exec(dag, "tiled")
     outs = []
     for _ in range(3):
         dA.copy_(orig)
@@ -163,3 +174,14 @@ def test_repeated_launches_identical(torch_gpu, oracle):
         ex.check()
         outs.append(dA.clone())
     assert all(bool((o == outs[0]).all()) for o in outs)
+
+
+def test_level_batching_stats(torch_gpu, oracle):
+    """Tiled mode runs one kernel per DAG level; LU with T tiles of size
+    <= 128 has 3 levels per step except the last (GETRF, TRSMs, GEMMs)."""
+    mu = _mu()
+    n, t = 1024, 8
+    dA = mu.to_device(oracle.gen_ref(n))
+    dag = mu.Dag().record_lu(mu.ptr(dA), n, t)
+    ex = mu.Exec(dag, "tiled")
+    assert ex.stats()["levels"] == 3 * (t - 1) + 1
