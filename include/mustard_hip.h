@@ -159,6 +159,8 @@ int mu_exec_launch(mu_exec_t exec, void *stream);
  * its item count (per-launch timing; a full run is every level in order). */
 int mu_exec_launch_level(mu_exec_t exec, int level, void *stream);
 int mu_exec_level_items(mu_exec_t exec, int level, long long *items);
+/* Items of one level per operation: counts[MU_OP_*] (6 entries). */
+int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);
 int mu_exec_stats(mu_exec_t exec, mu_exec_stats_t *stats);

@@ -175,6 +175,13 @@ class Exec:
     def launch_level(self, level, stream=None):
         call("mu_exec_launch_level", self.h, level, stream_handle(stream))
 
+    def level_ops(self, level):
+        """Items of one level per operation name."""
+        c = (ctypes.c_longlong * 6)()
+        call("mu_exec_level_ops", self.h, level, c)
+        names = ["NONE", "GEMM", "SYRK", "TRSM", "GETRF", "POTRF"]
+        return {names[i]: c[i] for i in range(6) if c[i]}
+
     def level_items(self, level):
         n = ctypes.c_longlong()
         call("mu_exec_level_items", self.h, level, ctypes.byref(n))

@@ -549,6 +549,29 @@ int mu_exec_level_items(mu_exec_t exec, int level, long long *items)
     return MU_OK;
 }
 
+int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts)
+{
+    const mu::Exec *e = exec ? exec->e : nullptr;
+    if (!e || !counts || level < 0 || level >= e->levels)
+        return fail(MU_EINVAL, "mu_exec_level_ops: bad arguments");
+    for (int i = 0; i < 6; i++)
+        counts[i] = 0;
+    for (size_t k = e->level_off[level]; k < e->level_off[level + 1]; k++) {
+        const uint32_t item = e->level_items[k];
+        // units are numbered in item order: find the unit of this item
+        size_t lo = 0, hi = e->low.units.size();
+        while (hi - lo > 1) {
+            size_t mid = (lo + hi) / 2;
+            if (e->low.units[mid].item_base <= item)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        counts[e->low.units[lo].op % 6]++;
+    }
+    return MU_OK;
+}
+
 int mu_exec_check(mu_exec_t exec)
 {
     if (!exec)

@@ -154,8 +154,11 @@ __device__ __attribute__((always_inline)) inline void gemm_block(const UnitDesc 
         for (int y = 0; y < 4; y++)
             acc[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    double ra[8], rb[8];
-    auto load = [&](int k0) {
+    // Global loads run two k-blocks ahead of the MFMAs: block kb+2 is fetched
+    // while block kb is computed and block kb+1 waits in registers for its
+    // LDS buffer.
+    double ra0[8], rb0[8], ra1[8], rb1[8];
+    auto load = [&](double (&ra)[8], double (&rb)[8], int k0) {
         const int kc = K - k0;
         if (TA)
             ld_kcontig<VEC>(ra, A + k0, u.lda, rows, kc);
@@ -166,7 +169,7 @@ __device__ __attribute__((always_inline)) inline void gemm_block(const UnitDesc 
         else
             ld_kcontig<VEC>(rb, B + k0, u.ldb, cols, kc);
     };
-    auto store = [&](double *As, double *Bs) {
+    auto store = [&](const double (&ra)[8], const double (&rb)[8], double *As, double *Bs) {
         if (TA)
             st_kcontig(As, ra);
         else
@@ -176,17 +179,8 @@ __device__ __attribute__((always_inline)) inline void gemm_block(const UnitDesc 
         else
             st_kcontig(Bs, rb);
     };
-
-    const int nk = (K + BK - 1) / BK;
-    load(0);
-    store(As0, Bs0);
-    __syncthreads();
     const int li = lane & 15, lk = lane >> 4;
-    for (int kb = 0; kb < nk; kb++) {
-        const double *As = (kb & 1) ? As1 : As0;
-        const double *Bs = (kb & 1) ? Bs1 : Bs0;
-        if (kb + 1 < nk)
-            load((kb + 1) * BK);
+    auto compute = [&](const double *As, const double *Bs) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             double fa[4], fb[4];
@@ -202,30 +196,60 @@ __device__ __attribute__((always_inline)) inline void gemm_block(const UnitDesc 
                 for (int y = 0; y < 4; y++)
                     acc[x][y] = mfma_f64(fa[x], fb[y], acc[x][y]);
         }
-        if (kb + 1 < nk) {
-            if (kb & 1)
-                store(As0, Bs0);
-            else
-                store(As1, Bs1);
-        }
+    };
+
+    const int nk = (K + BK - 1) / BK;
+    load(ra0, rb0, 0);
+    store(ra0, rb0, As0, Bs0);
+    if (nk > 1)
+        load(ra1, rb1, BK);
+    __syncthreads();
+    for (int kb = 0; kb < nk; kb += 2) {
+        if (kb + 2 < nk)
+            load(ra0, rb0, (kb + 2) * BK);
+        compute(As0, Bs0);
+        if (kb + 1 < nk)
+            store(ra1, rb1, As1, Bs1);
+        __syncthreads();
+        if (kb + 1 >= nk)
+            break;
+        if (kb + 3 < nk)
+            load(ra1, rb1, (kb + 3) * BK);
+        compute(As1, Bs1);
+        if (kb + 2 < nk)
+            store(ra0, rb0, As0, Bs0);
         __syncthreads();
     }
 
+    // Epilogue in four row-block chunks: the 16 C values of a chunk are all
+    // loaded before any is combined and stored, so their latencies overlap.
     const double alpha = u.alpha, beta = u.beta;
 #pragma unroll
-    for (int x = 0; x < 4; x++)
+    for (int x = 0; x < 4; x++) {
+        double cv[4][4];
+        bool ok[4][4];
 #pragma unroll
         for (int y = 0; y < 4; y++)
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const int r = wm * 64 + x * 16 + 4 * e + lk;
                 const int c = wn * 64 + y * 16 + li;
-                if (r < rows && c < cols && (!lower_only || i0 + r >= j0 + c)) {
-                    MU_G double *p = gp(u.c) + (i0 + r) + (size_t)(j0 + c) * u.ldc;
-                    const double old = beta == 0.0 ? 0.0 : beta * *p;
-                    *p = old + alpha * acc[x][y][e];
-                }
+                ok[y][e] = r < rows && c < cols && (!lower_only || i0 + r >= j0 + c);
+                cv[y][e] = (ok[y][e] && beta != 0.0)
+                               ? gp(u.c)[(i0 + r) + (size_t)(j0 + c) * u.ldc]
+                               : 0.0;
             }
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = wm * 64 + x * 16 + 4 * e + lk;
+                const int c = wn * 64 + y * 16 + li;
+                if (ok[y][e])
+                    gp(u.c)[(i0 + r) + (size_t)(j0 + c) * u.ldc] =
+                        beta * cv[y][e] + alpha * acc[x][y][e];
+            }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -280,8 +304,13 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
     const int li = lane & 15, lk = lane >> 4;
     for (int jb = 0; jb < n; jb += 16) {
         {
+            // diagonal block, with the reciprocal of each diagonal entry in
+            // place of the entry (the solve multiplies instead of dividing)
             const int r = tid >> 4, c = tid & 15;
-            Td[r * 17 + c] = (jb + r < n && c <= r) ? tv(jb + r, jb + c) : 0.0;
+            double v = (jb + r < n && c <= r) ? tv(jb + r, jb + c) : 0.0;
+            if (!UNIT && c == r && jb + r < n)
+                v = 1.0 / v;
+            Td[r * 17 + c] = v;
         }
         __syncthreads();
         if (tid < 16) {
@@ -298,7 +327,7 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
                     for (int q = 0; q < r; q++)
                         s -= Td[r * 17 + q] * y[q];
                     if (!UNIT)
-                        s /= Td[r * 17 + r];
+                        s *= Td[r * 17 + r];
                     y[r] = s;
                 }
             }
@@ -358,7 +387,6 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
 // ---------------------------------------------------------------------------
 // Small factorizations of an n x n tile (n <= SMALL_MAX) held in LDS.
 // ---------------------------------------------------------------------------
-constexpr int SP = SMALL_MAX + 1;  // LDS pitch (column-major copy S[c*SP + r])
 
 __device__ __forceinline__ void report_info(const UnitDesc &u, int col)
 {
@@ -367,77 +395,163 @@ __device__ __forceinline__ void report_info(const UnitDesc &u, int col)
                                __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The tile is held in registers: thread t owns rows (t % 16) + 16a and
+// columns (t / 16) + 16b, a, b = 0..7 (a cyclic 8 x 8 share of a 128 x 128
+// tile).  Each column step publishes the pivot row and the scaled pivot column
+// through LDS (double-buffered by step parity, two barriers per step) and
+// every thread applies the rank-1 update to its 64 elements.  Entries of the
+// published row/column at or before the pivot are zero, so the update leaves
+// finished rows and columns unchanged without any per-element test.
+struct SmallTile {
+    double r[8][8];
+};
+
+__device__ __forceinline__ void small_load(SmallTile &T, const MU_G double *a, size_t lda, int n,
+                                           int ti, int tj, bool lower_only)
+{
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            const int row = ti + 16 * x, col = tj + 16 * y;
+            T.r[x][y] = (row < n && col < n && (!lower_only || row >= col))
+                            ? a[row + col * lda]
+                            : 0.0;
+        }
+}
+
+__device__ __forceinline__ void small_store(const SmallTile &T, MU_G double *a, size_t lda, int n,
+                                            int ti, int tj, bool lower_only)
+{
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            const int row = ti + 16 * x, col = tj + 16 * y;
+            if (row < n && col < n && (!lower_only || row >= col))
+                a[row + col * lda] = T.r[x][y];
+        }
+}
+
 __device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc &u, double *smem)
 {
-    const int n = u.m, tid = threadIdx.x;
-    double *S = smem;
+    const int n = u.m, tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
     MU_G double *a = gp(u.c);
     const size_t lda = u.ldc;
-    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
-        const int r = idx % n, c = idx / n;
-        S[c * SP + r] = a[r + c * lda];
-    }
-    __syncthreads();
+    double *rowbuf = smem;        // [2][128]
+    double *colbuf = smem + 256;  // [2][128]
+    double *pivbuf = smem + 512;  // [2]
+    SmallTile T;
+    small_load(T, a, lda, n, ti, tj, false);
     for (int c = 0; c < n; c++) {
-        const double piv = S[c * SP + c];
-        if (tid == 0 && piv == 0.0)
-            report_info(u, c);
-        for (int r = c + 1 + tid; r < n; r += WG_THREADS)
-            S[c * SP + r] /= piv;
-        __syncthreads();
-        const int m = n - c - 1;
-        for (int e = tid; e < m * m; e += WG_THREADS) {
-            const int r = c + 1 + e % m, cc = c + 1 + e / m;
-            S[cc * SP + r] -= S[c * SP + r] * S[cc * SP + c];
+        const int p = c & 1, own = c & 15, blk = c >> 4;
+        if (ti == own) {
+            // owners of row c publish its entries right of the pivot
+#pragma unroll
+            for (int x = 0; x < 8; x++)
+                if (x == blk)
+#pragma unroll
+                    for (int y = 0; y < 8; y++) {
+                        const int col = tj + 16 * y;
+                        rowbuf[p * 128 + col] = col > c ? T.r[x][y] : 0.0;
+                        if (col == c) {
+                            pivbuf[p] = T.r[x][y];
+                            if (T.r[x][y] == 0.0)
+                                report_info(u, c);
+                        }
+                    }
         }
         __syncthreads();
+        if (tj == own) {
+            // owners of column c scale it below the pivot (by the reciprocal,
+            // as LAPACK's dgetf2 does) and publish it
+            const double rp = 1.0 / pivbuf[p];
+#pragma unroll
+            for (int y = 0; y < 8; y++)
+                if (y == blk)
+#pragma unroll
+                    for (int x = 0; x < 8; x++) {
+                        const int row = ti + 16 * x;
+                        if (row > c)
+                            T.r[x][y] = T.r[x][y] * rp;
+                        colbuf[p * 128 + row] = row > c ? T.r[x][y] : 0.0;
+                    }
+        }
+        __syncthreads();
+        double lv[8], uv[8];
+#pragma unroll
+        for (int x = 0; x < 8; x++)
+            lv[x] = colbuf[p * 128 + ti + 16 * x];
+#pragma unroll
+        for (int y = 0; y < 8; y++)
+            uv[y] = rowbuf[p * 128 + tj + 16 * y];
+#pragma unroll
+        for (int x = 0; x < 8; x++)
+#pragma unroll
+            for (int y = 0; y < 8; y++)
+                T.r[x][y] = fma(-lv[x], uv[y], T.r[x][y]);
     }
-    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
-        const int r = idx % n, c = idx / n;
-        a[r + c * lda] = S[c * SP + r];
-    }
+    small_store(T, a, lda, n, ti, tj, false);
 }
 
 __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc &u, double *smem)
 {
-    const int n = u.m, tid = threadIdx.x;
-    double *S = smem;
+    const int n = u.m, tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
     MU_G double *a = gp(u.c);
     const size_t lda = u.ldc;
-    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
-        const int r = idx % n, c = idx / n;
-        if (r >= c)
-            S[c * SP + r] = a[r + c * lda];
-    }
-    __syncthreads();
+    double *colbuf = smem + 256;  // [2][128]
+    double *pivbuf = smem + 512;  // [2]
+    SmallTile T;
+    small_load(T, a, lda, n, ti, tj, true);
     for (int c = 0; c < n; c++) {
-        double d = S[c * SP + c];
-        if (tid == 0 && !(d > 0.0))
-            report_info(u, c);
-        d = sqrt(d);
-        __syncthreads();
-        for (int r = c + tid; r < n; r += WG_THREADS)
-            S[c * SP + r] = (r == c) ? d : S[c * SP + r] / d;
-        __syncthreads();
-        const int m = n - c - 1;
-        for (int e = tid; e < m * m; e += WG_THREADS) {
-            const int r = c + 1 + e % m, cc = c + 1 + e / m;
-            if (r >= cc)
-                S[cc * SP + r] -= S[c * SP + r] * S[c * SP + cc];
+        const int p = c & 1, own = c & 15, blk = c >> 4;
+        if (ti == own && tj == own) {
+#pragma unroll
+            for (int x = 0; x < 8; x++)
+                if (x == blk) {
+                    const double v = T.r[x][x];
+                    if (!(v > 0.0))
+                        report_info(u, c);
+                    pivbuf[p] = sqrt(v);
+                }
         }
         __syncthreads();
+        if (tj == own) {
+            const double d = pivbuf[p], rd = 1.0 / d;
+#pragma unroll
+            for (int y = 0; y < 8; y++)
+                if (y == blk)
+#pragma unroll
+                    for (int x = 0; x < 8; x++) {
+                        const int row = ti + 16 * x;
+                        if (row > c)
+                            T.r[x][y] = T.r[x][y] * rd;
+                        else if (row == c)
+                            T.r[x][y] = d;
+                        colbuf[p * 128 + row] = row > c ? T.r[x][y] : 0.0;
+                    }
+        }
+        __syncthreads();
+        double lv[8], lw[8];
+#pragma unroll
+        for (int x = 0; x < 8; x++)
+            lv[x] = colbuf[p * 128 + ti + 16 * x];
+#pragma unroll
+        for (int y = 0; y < 8; y++)
+            lw[y] = colbuf[p * 128 + tj + 16 * y];
+#pragma unroll
+        for (int x = 0; x < 8; x++)
+#pragma unroll
+            for (int y = 0; y < 8; y++)
+                T.r[x][y] = fma(-lv[x], lw[y], T.r[x][y]);
     }
-    for (int idx = tid; idx < n * n; idx += WG_THREADS) {
-        const int r = idx % n, c = idx / n;
-        if (r >= c)
-            a[r + c * lda] = S[c * SP + r];
-    }
+    small_store(T, a, lda, n, ti, tj, true);
 }
 
 // LDS (doubles) one workgroup needs for any item.
 constexpr int LDS_GEMM = 2 * (LDS_I > LDS_K ? LDS_I : LDS_K) * 2;
 constexpr int LDS_TRSM = TRSM_NMAX * 16 + 16 * 17;
-constexpr int LDS_SMALL = SMALL_MAX * SP;
+constexpr int LDS_SMALL = 516;
 constexpr int LDS_DOUBLES = LDS_GEMM > LDS_TRSM ? (LDS_GEMM > LDS_SMALL ? LDS_GEMM : LDS_SMALL)
                                                 : (LDS_TRSM > LDS_SMALL ? LDS_TRSM : LDS_SMALL);
 
