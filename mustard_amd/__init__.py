@@ -182,6 +182,12 @@ class Exec:
         names = ["NONE", "GEMM", "SYRK", "TRSM", "GETRF", "POTRF"]
         return {names[i]: c[i] for i in range(6) if c[i]}
 
+    def level_info(self, level):
+        """(kernel name, algorithmic flops) of one level of a tiled executable."""
+        k, f = ctypes.c_int(), ctypes.c_double()
+        call("mu_exec_level_info", self.h, level, ctypes.byref(k), ctypes.byref(f))
+        return ("k_batch_gemm" if k.value == 1 else "k_batch"), f.value
+
     def level_items(self, level):
         n = ctypes.c_longlong()
         call("mu_exec_level_items", self.h, level, ctypes.byref(n))

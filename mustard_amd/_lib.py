@@ -85,6 +85,8 @@ SIGNATURES = {
     "mu_exec_launch_level": (_i, [_vp, _i, _vp]),
     "mu_exec_level_items": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong)]),
     "mu_exec_level_ops": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_longlong)]),
+    "mu_exec_level_info": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_double)]),
     "mu_exec_stats": (_i, [_vp, ctypes.POINTER(ExecStats)]),
     "mu_exec_trace": (_i, [_vp, ctypes.POINTER(ctypes.c_ulonglong), _sz]),
     "mu_exec_item_units": (_i, [_vp, _ip, _ip, _ip]),

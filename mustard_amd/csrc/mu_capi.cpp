@@ -572,6 +572,18 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts)
     return MU_OK;
 }
 
+int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops)
+{
+    const mu::Exec *e = exec ? exec->e : nullptr;
+    if (!e || level < 0 || level >= e->levels)
+        return fail(MU_EINVAL, "mu_exec_level_info: tiled executables only, level out of range");
+    if (kernel)
+        *kernel = e->level_gemm[level] ? MU_KERNEL_GEMM : MU_KERNEL_GENERIC;
+    if (flops)
+        *flops = e->level_flops[level];
+    return MU_OK;
+}
+
 int mu_exec_check(mu_exec_t exec)
 {
     if (!exec)

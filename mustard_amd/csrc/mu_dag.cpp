@@ -234,7 +234,8 @@ struct Builder {
             UnitDesc d = base(op);
             d.gm = cdiv(op.m, GEMM_BM);
             d.gn = cdiv(op.n, GEMM_BN);
-            if (aligned16(op.a) && aligned16(op.b) && op.lda % 2 == 0 && op.ldb % 2 == 0)
+            if (aligned16(op.a) && aligned16(op.b) && op.lda % 2 == 0 && op.ldb % 2 == 0 &&
+                op.m % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;
             int u = add_unit(d);
             f.src = {u};
@@ -251,7 +252,7 @@ struct Builder {
             d.gm = cdiv(op.n, GEMM_BM);
             d.gn = d.gm;
             d.flags |= F_TB;
-            if (aligned16(op.a) && op.lda % 2 == 0)
+            if (aligned16(op.a) && op.lda % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;
             int u = add_unit(d);
             f.src = {u};

@@ -30,6 +30,18 @@ __device__ __forceinline__ MU_G T *gp(T *p)
     return (MU_G T *)p;
 }
 
+// Workgroup barrier for LDS hand-offs only.  __syncthreads() is a
+// workgroup-scope fence plus s_barrier, and the fence waits for every
+// outstanding global load as well (vmcnt(0)), which would drain the
+// prefetches the loops below keep in flight across their barriers.  No item
+// body passes data between waves through global memory, so waiting for the
+// LDS operations (lgkmcnt(0)) before the barrier is enough; the "memory"
+// clobber keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // v_mfma_f64_16x16x4_f64.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
 // it receives D[4*e + l/16][l%16] in element e (register e holds rows 4e..4e+3).
 __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c)
@@ -38,84 +50,86 @@ __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c)
 }
 
 // ---------------------------------------------------------------------------
-// GEMM / SYRK output block: C[i0:i0+128, j0:j0+128] (+)= alpha op(A) op(B)
+// GEMM / SYRK output block: C[i0:i0+64, j0:j0+64] = beta C + alpha op(A) op(B)
 // ---------------------------------------------------------------------------
+// Four waves in a 2 x 2 grid, each owning a 32 x 32 sub-block (2 x 2 MFMA
+// tiles).  K runs in blocks of 8, double-buffered in LDS.  The kernel that
+// runs GEMM-only batches needs 20 KB of LDS and at most 128 VGPRs per lane,
+// so four workgroups share a CU and their waves hide each other's latency.
+//
 // LDS layouts.  An operand whose stored columns run along the output
 // dimension ("i-contiguous": A, or B^T) is kept as S[k][i] with a pitch of
 // PI doubles; one whose stored columns run along k ("k-contiguous": A^T, or B)
-// is kept as S[i][k] with a pitch of PK doubles.  Both pitches make the
-// fragment reads of a half-wave fall in distinct banks.
+// is kept as S[i][k] with a pitch of PK doubles.
+static_assert(GEMM_BM == 64 && GEMM_BN == 64 && GEMM_BK == 8, "GEMM loaders assume 64 x 64 x 8");
 constexpr int PI = GEMM_BM + 16;
 constexpr int PK = GEMM_BK + 1;
 constexpr int LDS_I = GEMM_BK * PI;   // doubles per i-contiguous buffer
 constexpr int LDS_K = GEMM_BM * PK;   // doubles per k-contiguous buffer
 
-// Global -> registers, i-contiguous operand: element (r, k) at p[r + k*ld],
-// r < rows, k < kcols.  Thread t covers rows 2(t%64), 2(t%64)+1 of columns
-// t/64 + 4q, q = 0..3.
+// The loaders below issue every load unconditionally, from an address clamped
+// into the operand; the values that fall outside it are zeroed only when the
+// registers are written to LDS.  With no branch around a load and no use of
+// its result before the LDS store, the compiler's wait counting stays exact
+// and a prefetch stays in flight until the half-step that stores it.  VEC
+// (16-byte aligned operands, even leading dimensions and even m, n, k, set by
+// the lowering) loads element pairs, which then never straddle an edge.
+
+// i-contiguous operand: element (r, k) at p[r + k*ld], r < rows, k < kcols.
+// Thread t covers rows 2(t%32), 2(t%32)+1 of column t/32.
 template <bool VEC>
-__device__ __forceinline__ void ld_icontig(double (&v)[8], const MU_G double *p, int ld, int rows,
+__device__ __forceinline__ void ld_icontig(double (&v)[2], const MU_G double *p, int ld, int rows,
                                            int kcols)
 {
-    const int t = threadIdx.x, r = (t & 63) * 2, c0 = t >> 6;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int c = c0 + 4 * q;
-        const MU_G double *s = p + r + (size_t)c * ld;
-        if (VEC && c < kcols && r + 1 < rows) {
-            const v2d x = *reinterpret_cast<const MU_G v2d *>(s);
-            v[2 * q] = x[0];
-            v[2 * q + 1] = x[1];
-        } else {
-            v[2 * q] = (c < kcols && r < rows) ? s[0] : 0.0;
-            v[2 * q + 1] = (c < kcols && r + 1 < rows) ? s[1] : 0.0;
-        }
+    const int t = threadIdx.x, r = (t & 31) * 2, c = t >> 5;
+    const int cc = max(0, min(c, kcols - 1));
+    if (VEC) {
+        const int rr = max(0, min(r, rows - 2));
+        const v2d x = *reinterpret_cast<const MU_G v2d *>(p + rr + (size_t)cc * ld);
+        v[0] = x[0];
+        v[1] = x[1];
+    } else {
+        v[0] = p[min(r, rows - 1) + (size_t)cc * ld];
+        v[1] = p[min(r + 1, rows - 1) + (size_t)cc * ld];
     }
 }
 
-__device__ __forceinline__ void st_icontig(double *S, const double (&v)[8])
+__device__ __forceinline__ void st_icontig(double *S, const double (&v)[2], int rows, int kcols)
 {
-    const int t = threadIdx.x, r = (t & 63) * 2, c0 = t >> 6;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        v2d x;
-        x[0] = v[2 * q];
-        x[1] = v[2 * q + 1];
-        *reinterpret_cast<v2d *>(S + (c0 + 4 * q) * PI + r) = x;
-    }
+    const int t = threadIdx.x, r = (t & 31) * 2, c = t >> 5;
+    const bool cin = c < kcols;
+    v2d x;
+    x[0] = (cin && r < rows) ? v[0] : 0.0;
+    x[1] = (cin && r + 1 < rows) ? v[1] : 0.0;
+    *reinterpret_cast<v2d *>(S + c * PI + r) = x;
 }
 
-// Global -> registers, k-contiguous operand: element (r, k) at p[k + r*ld].
-// Thread t covers k = 2(t%8), 2(t%8)+1 of rows t/8 + 32q, q = 0..3.
+// k-contiguous operand: element (r, k) at p[k + r*ld].
+// Thread t covers k = 2(t%4), 2(t%4)+1 of row t/4.
 template <bool VEC>
-__device__ __forceinline__ void ld_kcontig(double (&v)[8], const MU_G double *p, int ld, int rows,
+__device__ __forceinline__ void ld_kcontig(double (&v)[2], const MU_G double *p, int ld, int rows,
                                            int kcols)
 {
-    const int t = threadIdx.x, k = (t & 7) * 2, r0 = t >> 3;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int r = r0 + 32 * q;
-        const MU_G double *s = p + k + (size_t)r * ld;
-        if (VEC && r < rows && k + 1 < kcols) {
-            const v2d x = *reinterpret_cast<const MU_G v2d *>(s);
-            v[2 * q] = x[0];
-            v[2 * q + 1] = x[1];
-        } else {
-            v[2 * q] = (r < rows && k < kcols) ? s[0] : 0.0;
-            v[2 * q + 1] = (r < rows && k + 1 < kcols) ? s[1] : 0.0;
-        }
+    const int t = threadIdx.x, k = (t & 3) * 2, r = t >> 2;
+    const int rr = min(r, rows - 1);
+    if (VEC) {
+        const int kk = max(0, min(k, kcols - 2));
+        const v2d x = *reinterpret_cast<const MU_G v2d *>(p + kk + (size_t)rr * ld);
+        v[0] = x[0];
+        v[1] = x[1];
+    } else {
+        v[0] = p[max(0, min(k, kcols - 1)) + (size_t)rr * ld];
+        v[1] = p[max(0, min(k + 1, kcols - 1)) + (size_t)rr * ld];
     }
 }
 
-__device__ __forceinline__ void st_kcontig(double *S, const double (&v)[8])
+__device__ __forceinline__ void st_kcontig(double *S, const double (&v)[2], int rows, int kcols)
 {
-    const int t = threadIdx.x, k = (t & 7) * 2, r0 = t >> 3;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        double *d = S + (r0 + 32 * q) * PK + k;
-        d[0] = v[2 * q];
-        d[1] = v[2 * q + 1];
-    }
+    const int t = threadIdx.x, k = (t & 3) * 2, r = t >> 2;
+    const bool rin = r < rows;
+    double *d = S + r * PK + k;
+    d[0] = (rin && k < kcols) ? v[0] : 0.0;
+    d[1] = (rin && k + 1 < kcols) ? v[1] : 0.0;
 }
 
 // op(A)(i, k) read from the LDS copy.
@@ -138,7 +152,7 @@ __device__ __attribute__((always_inline)) inline void gemm_block(const UnitDesc 
     constexpr int SA = TA ? LDS_K : LDS_I;
     constexpr int SB = TB ? LDS_I : LDS_K;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave & 1, wn = wave >> 1;  // 2 x 2 waves of 64 x 64
+    const int wm = wave & 1, wn = wave >> 1;  // 2 x 2 waves of 32 x 32
     const int i0 = bi * BM, j0 = bj * BN;
     const int rows = min(BM, u.m - i0), cols = min(BN, u.n - j0);
     const int K = u.k;
@@ -147,109 +161,113 @@ __device__ __attribute__((always_inline)) inline void gemm_block(const UnitDesc 
     const MU_G double *B = TB ? gp(u.b) + j0 : gp(u.b) + (size_t)j0 * u.ldb;
     double *As0 = smem, *As1 = smem + SA, *Bs0 = smem + 2 * SA, *Bs1 = smem + 2 * SA + SB;
 
-    v4d acc[4][4];
+    v4d acc[2][2];
 #pragma unroll
-    for (int x = 0; x < 4; x++)
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int y = 0; y < 4; y++)
+        for (int y = 0; y < 2; y++)
             acc[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
 
     // Global loads run two k-blocks ahead of the MFMAs: block kb+2 is fetched
     // while block kb is computed and block kb+1 waits in registers for its
     // LDS buffer.
-    double ra0[8], rb0[8], ra1[8], rb1[8];
-    auto load = [&](double (&ra)[8], double (&rb)[8], int k0) {
+    double ra0[2], rb0[2], ra1[2], rb1[2];
+    auto load = [&](double (&ra)[2], double (&rb)[2], int k0) {
+        // a block past K is read (and zeroed) at k-offset 0, inside the operand
+        const int kc = K - k0, ks = kc > 0 ? k0 : 0;
+        if (TA)
+            ld_kcontig<VEC>(ra, A + ks, u.lda, rows, kc);
+        else
+            ld_icontig<VEC>(ra, A + (size_t)ks * u.lda, u.lda, rows, kc);
+        if (TB)
+            ld_icontig<VEC>(rb, B + (size_t)ks * u.ldb, u.ldb, cols, kc);
+        else
+            ld_kcontig<VEC>(rb, B + ks, u.ldb, cols, kc);
+    };
+    // store the block loaded for k-offset k0 (zeroing what lies outside)
+    auto store = [&](const double (&ra)[2], const double (&rb)[2], double *As, double *Bs,
+                     int k0) {
         const int kc = K - k0;
         if (TA)
-            ld_kcontig<VEC>(ra, A + k0, u.lda, rows, kc);
+            st_kcontig(As, ra, rows, kc);
         else
-            ld_icontig<VEC>(ra, A + (size_t)k0 * u.lda, u.lda, rows, kc);
+            st_icontig(As, ra, rows, kc);
         if (TB)
-            ld_icontig<VEC>(rb, B + (size_t)k0 * u.ldb, u.ldb, cols, kc);
+            st_icontig(Bs, rb, cols, kc);
         else
-            ld_kcontig<VEC>(rb, B + k0, u.ldb, cols, kc);
-    };
-    auto store = [&](const double (&ra)[8], const double (&rb)[8], double *As, double *Bs) {
-        if (TA)
-            st_kcontig(As, ra);
-        else
-            st_icontig(As, ra);
-        if (TB)
-            st_icontig(Bs, rb);
-        else
-            st_kcontig(Bs, rb);
+            st_kcontig(Bs, rb, cols, kc);
     };
     const int li = lane & 15, lk = lane >> 4;
     auto compute = [&](const double *As, const double *Bs) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            double fa[4], fb[4];
+            double fa[2], fb[2];
 #pragma unroll
-            for (int x = 0; x < 4; x++)
-                fa[x] = frag_a<TA>(As, wm * 64 + x * 16 + li, kk + lk);
+            for (int x = 0; x < 2; x++)
+                fa[x] = frag_a<TA>(As, wm * 32 + x * 16 + li, kk + lk);
 #pragma unroll
-            for (int y = 0; y < 4; y++)
-                fb[y] = frag_b<TB>(Bs, kk + lk, wn * 64 + y * 16 + li);
+            for (int y = 0; y < 2; y++)
+                fb[y] = frag_b<TB>(Bs, kk + lk, wn * 32 + y * 16 + li);
 #pragma unroll
-            for (int x = 0; x < 4; x++)
+            for (int x = 0; x < 2; x++)
 #pragma unroll
-                for (int y = 0; y < 4; y++)
+                for (int y = 0; y < 2; y++)
                     acc[x][y] = mfma_f64(fa[x], fb[y], acc[x][y]);
         }
     };
 
-    const int nk = (K + BK - 1) / BK;
+    // The k-block count is rounded up to even; a block past K loads as zeros
+    // and adds nothing.  Loads past the last block read clamped addresses and
+    // are never used.
+    const int nk = ((K + BK - 1) / BK + 1) & ~1;
+    if (K <= 0)
+        goto epilogue;  // C = beta C: the operands are not read at all
     load(ra0, rb0, 0);
-    store(ra0, rb0, As0, Bs0);
-    if (nk > 1)
-        load(ra1, rb1, BK);
-    __syncthreads();
+    store(ra0, rb0, As0, Bs0, 0);
+    load(ra1, rb1, BK);
+    lds_barrier();
     for (int kb = 0; kb < nk; kb += 2) {
-        if (kb + 2 < nk)
-            load(ra0, rb0, (kb + 2) * BK);
+        load(ra0, rb0, (kb + 2) * BK);
         compute(As0, Bs0);
-        if (kb + 1 < nk)
-            store(ra1, rb1, As1, Bs1);
-        __syncthreads();
-        if (kb + 1 >= nk)
-            break;
-        if (kb + 3 < nk)
-            load(ra1, rb1, (kb + 3) * BK);
+        store(ra1, rb1, As1, Bs1, (kb + 1) * BK);
+        lds_barrier();
+        load(ra1, rb1, (kb + 3) * BK);
         compute(As1, Bs1);
-        if (kb + 2 < nk)
-            store(ra0, rb0, As0, Bs0);
-        __syncthreads();
+        store(ra0, rb0, As0, Bs0, (kb + 2) * BK);
+        lds_barrier();
     }
 
-    // Epilogue in four row-block chunks: the 16 C values of a chunk are all
-    // loaded before any is combined and stored, so their latencies overlap.
+epilogue:
+    // Epilogue: all 16 C values of a lane are loaded before any is combined
+    // and stored, so their latencies overlap.
     const double alpha = u.alpha, beta = u.beta;
+    double cv[2][2][4];
+    bool ok[2][2][4];
 #pragma unroll
-    for (int x = 0; x < 4; x++) {
-        double cv[4][4];
-        bool ok[4][4];
+    for (int x = 0; x < 2; x++)
 #pragma unroll
-        for (int y = 0; y < 4; y++)
+        for (int y = 0; y < 2; y++)
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const int r = wm * 64 + x * 16 + 4 * e + lk;
-                const int c = wn * 64 + y * 16 + li;
-                ok[y][e] = r < rows && c < cols && (!lower_only || i0 + r >= j0 + c);
-                cv[y][e] = (ok[y][e] && beta != 0.0)
-                               ? gp(u.c)[(i0 + r) + (size_t)(j0 + c) * u.ldc]
-                               : 0.0;
+                const int r = wm * 32 + x * 16 + 4 * e + lk;
+                const int c = wn * 32 + y * 16 + li;
+                ok[x][y][e] = r < rows && c < cols && (!lower_only || i0 + r >= j0 + c);
+                cv[x][y][e] = (ok[x][y][e] && beta != 0.0)
+                                  ? gp(u.c)[(i0 + r) + (size_t)(j0 + c) * u.ldc]
+                                  : 0.0;
             }
 #pragma unroll
-        for (int y = 0; y < 4; y++)
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const int r = wm * 64 + x * 16 + 4 * e + lk;
-                const int c = wn * 64 + y * 16 + li;
-                if (ok[y][e])
+                const int r = wm * 32 + x * 16 + 4 * e + lk;
+                const int c = wn * 32 + y * 16 + li;
+                if (ok[x][y][e])
                     gp(u.c)[(i0 + r) + (size_t)(j0 + c) * u.ldc] =
-                        beta * cv[y][e] + alpha * acc[x][y][e];
+                        beta * cv[x][y][e] + alpha * acc[x][y][e];
             }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -299,20 +317,38 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
         }
         Y[r * 16 + c] = (r < n && c < w) ? alpha * *zp(r, c) : 0.0;
     }
-    __syncthreads();
+    lds_barrier();
 
+    // Each 16-row step jb: stage the diagonal block (reciprocals of its
+    // diagonal in place of the entries, so the solve multiplies), solve it
+    // with one thread per right-hand side, then update the rows below,
+    // Y[rb] -= T[rb, jb] Y[jb], with FP64 MFMA.  The triangle fragments of
+    // step jb+16 are loaded during step jb, row block by row block right
+    // after that block's MFMAs, so they are consumed in the order they were
+    // issued and at most 64 loads per lane are in flight (the hardware's
+    // in-order vector-memory counter tracks 63).
     const int li = lane & 15, lk = lane >> 4;
-    for (int jb = 0; jb < n; jb += 16) {
-        {
-            // diagonal block, with the reciprocal of each diagonal entry in
-            // place of the entry (the solve multiplies instead of dividing)
-            const int r = tid >> 4, c = tid & 15;
-            double v = (jb + r < n && c <= r) ? tv(jb + r, jb + c) : 0.0;
-            if (!UNIT && c == r && jb + r < n)
-                v = 1.0 / v;
-            Td[r * 17 + c] = v;
-        }
-        __syncthreads();
+    const int r16 = tid >> 4, c16 = tid & 15;
+    constexpr int MAXB = TRSM_NMAX / 64;
+    // Loads are unconditional, from addresses clamped into the triangle;
+    // values outside it are zeroed where they are used (see the header of
+    // the GEMM loaders for why no load sits behind a branch).
+    auto load_diag = [&](int jb) -> double {
+        return tv(min(jb + r16, n - 1), min(jb + c16, n - 1));
+    };
+    auto load_block = [&](double (&f)[4], int jb, int i) {
+        const int rb = jb + 16 + 16 * wave + 64 * i;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+            f[kk] = tv(min(rb + li, n - 1), min(jb + 4 * kk + lk, n - 1));
+    };
+    auto step = [&](int jb, double dv, const double (&fa)[MAXB][4], double &dn,
+                    double (&fn)[MAXB][4]) {
+        dv = (jb + r16 < n && c16 <= r16) ? dv : 0.0;
+        if (!UNIT && c16 == r16 && jb + r16 < n)
+            dv = 1.0 / dv;
+        Td[r16 * 17 + c16] = dv;
+        lds_barrier();
         if (tid < 16) {
             const int nb = min(16, n - jb);
             double y[16];
@@ -336,20 +372,8 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
                 if (r < nb)
                     Y[(jb + r) * 16 + tid] = y[r];
         }
-        __syncthreads();
-        // Rows below the diagonal block: Y[rb] -= T[rb, jb] Y[jb].  All the
-        // triangle fragments of this wave are loaded before the MFMAs run.
-        constexpr int MAXB = TRSM_NMAX / 64;
-        double fa[MAXB][4];
-#pragma unroll
-        for (int i = 0; i < MAXB; i++) {
-            const int rb = jb + 16 + 16 * wave + 64 * i;
-#pragma unroll
-            for (int kk = 0; kk < 4; kk++) {
-                const int r = rb + li, c = jb + 4 * kk + lk;
-                fa[i][kk] = (rb < np && r < n && c < n) ? tv(r, c) : 0.0;
-            }
-        }
+        lds_barrier();
+        dn = load_diag(jb + 16);
         double fb[4];
 #pragma unroll
         for (int kk = 0; kk < 4; kk++)
@@ -360,14 +384,28 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
             if (rb < np) {
                 v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int kk = 0; kk < 4; kk++)
-                    acc = mfma_f64(fa[i][kk], fb[kk], acc);
+                for (int kk = 0; kk < 4; kk++) {
+                    const bool in = rb + li < n && jb + 4 * kk + lk < n;
+                    acc = mfma_f64(in ? fa[i][kk] : 0.0, fb[kk], acc);
+                }
 #pragma unroll
                 for (int e = 0; e < 4; e++)
                     Y[(rb + 4 * e + lk) * 16 + li] -= acc[e];
             }
+            load_block(fn[i], jb + 16, i);
         }
-        __syncthreads();
+        lds_barrier();
+    };
+    double fa0[MAXB][4], fa1[MAXB][4];
+    double d0 = load_diag(0), d1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < MAXB; i++)
+        load_block(fa0[i], 0, i);
+    for (int jb = 0; jb < n; jb += 32) {
+        step(jb, d0, fa0, d1, fa1);
+        if (jb + 16 >= n)
+            break;
+        step(jb + 16, d1, fa1, d0, fa0);
     }
 
     for (int idx = tid; idx < np * 16; idx += WG_THREADS) {
@@ -461,7 +499,7 @@ __device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc
                         }
                     }
         }
-        __syncthreads();
+        lds_barrier();
         if (tj == own) {
             // owners of column c scale it below the pivot (by the reciprocal,
             // as LAPACK's dgetf2 does) and publish it
@@ -477,7 +515,7 @@ __device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc
                         colbuf[p * 128 + row] = row > c ? T.r[x][y] : 0.0;
                     }
         }
-        __syncthreads();
+        lds_barrier();
         double lv[8], uv[8];
 #pragma unroll
         for (int x = 0; x < 8; x++)
@@ -515,7 +553,7 @@ __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc
                     pivbuf[p] = sqrt(v);
                 }
         }
-        __syncthreads();
+        lds_barrier();
         if (tj == own) {
             const double d = pivbuf[p], rd = 1.0 / d;
 #pragma unroll
@@ -531,7 +569,7 @@ __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc
                         colbuf[p * 128 + row] = row > c ? T.r[x][y] : 0.0;
                     }
         }
-        __syncthreads();
+        lds_barrier();
         double lv[8], lw[8];
 #pragma unroll
         for (int x = 0; x < 8; x++)
@@ -549,7 +587,7 @@ __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc
 }
 
 // LDS (doubles) one workgroup needs for any item.
-constexpr int LDS_GEMM = 2 * (LDS_I > LDS_K ? LDS_I : LDS_K) * 2;
+constexpr int LDS_GEMM = 2 * (LDS_I > LDS_K ? LDS_I : LDS_K) * 2;  // 2 operands x 2 buffers
 constexpr int LDS_TRSM = TRSM_NMAX * 16 + 16 * 17;
 constexpr int LDS_SMALL = 516;
 constexpr int LDS_DOUBLES = LDS_GEMM > LDS_TRSM ? (LDS_GEMM > LDS_SMALL ? LDS_GEMM : LDS_SMALL)
@@ -577,28 +615,33 @@ __device__ __forceinline__ void gemm_dispatch_vec(const UnitDesc &u, int bi, int
         gemm_block<TA, TB, false>(u, bi, bj, lo, smem);
 }
 
+// GEMM or SYRK item `it` of unit `u` (the only kinds k_batch_gemm runs).
+__device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDesc &u, int it,
+                                                                    double *smem)
+{
+    if (u.op == OP_SYRK) {
+        int bi, bj;
+        syrk_block(u.gm, it, bi, bj);
+        gemm_dispatch_vec<false, true>(u, bi, bj, bi == bj, smem);
+        return;
+    }
+    const int bi = it % u.gm, bj = it / u.gm;
+    switch (u.flags & (F_TA | F_TB)) {
+    case 0: gemm_dispatch_vec<false, false>(u, bi, bj, false, smem); break;
+    case F_TB: gemm_dispatch_vec<false, true>(u, bi, bj, false, smem); break;
+    case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
+    default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
+    }
+}
+
 // Execute local item `it` of unit `u`.  Always inlined, so that LDS stays
 // addressed as LDS and no call stack is needed in the persistent kernel.
 __device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u, int it,
                                                                double *smem)
 {
     switch (u.op) {
-    case OP_GEMM: {
-        const int bi = it % u.gm, bj = it / u.gm;
-        switch (u.flags & (F_TA | F_TB)) {
-        case 0: gemm_dispatch_vec<false, false>(u, bi, bj, false, smem); break;
-        case F_TB: gemm_dispatch_vec<false, true>(u, bi, bj, false, smem); break;
-        case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
-        default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
-        }
-        break;
-    }
-    case OP_SYRK: {
-        int bi, bj;
-        syrk_block(u.gm, it, bi, bj);
-        gemm_dispatch_vec<false, true>(u, bi, bj, bi == bj, smem);
-        break;
-    }
+    case OP_GEMM:
+    case OP_SYRK: run_gemm_item(u, it, smem); break;
     case OP_TRSM:
         switch ((u.flags >> 3) & 0x27) {  // T_TRANS, Z_ROWS, UNIT, REV
 #define MU_TRSM_CASE(bits)                                                                  \

@@ -29,17 +29,27 @@ namespace mu {
 #define AGENT __HIP_MEMORY_SCOPE_AGENT
 
 // ---------------------------------------------------------------------------
-// per-unit grid kernel (graph and stream modes): item = blockIdx.x
+// per-unit grid kernels (graph and stream modes): item = blockIdx.x.  The
+// GEMM/SYRK variants need only the GEMM block's LDS, so four workgroups fit
+// on a CU (the generic kernel's LDS, sized for the TRSM panel, allows one).
 // ---------------------------------------------------------------------------
+#define MU_GEMM_KERNEL __launch_bounds__(WG_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
+
 __global__ __launch_bounds__(WG_THREADS) void k_unit(UnitDesc d)
 {
     __shared__ double smem[LDS_DOUBLES];
     run_item(d, blockIdx.x, smem);
 }
 
+__global__ MU_GEMM_KERNEL void k_unit_gemm(UnitDesc d)
+{
+    __shared__ double smem[LDS_GEMM];
+    run_gemm_item(d, blockIdx.x, smem);
+}
+
 // ---------------------------------------------------------------------------
-// level-batched kernel (graph mode): one launch runs every item of every unit
-// at one level of the unit DAG; block b runs items[b]
+// level-batched kernels (graph mode): one launch runs every item of every
+// unit at one level of the unit DAG; block b runs items[b]
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(WG_THREADS) void k_batch(const UnitDesc *units,
                                                       const uint32_t *item_unit,
@@ -47,9 +57,35 @@ __global__ __launch_bounds__(WG_THREADS) void k_batch(const UnitDesc *units,
 {
     __shared__ double smem[LDS_DOUBLES];
     const uint32_t item = items[blockIdx.x];
-    const uint32_t u = item_unit[item];
-    const UnitDesc *d = units + u;
-    run_item(*d, (int)(item - d->item_base), smem);
+    // a private copy of the descriptor lives in registers; a reference into
+    // global memory would be re-read after every barrier of the item body
+    const UnitDesc d = units[item_unit[item]];
+    run_item(d, (int)(item - d.item_base), smem);
+}
+
+__global__ MU_GEMM_KERNEL void k_batch_gemm(const UnitDesc *units, const uint32_t *item_unit,
+                                            const uint32_t *items)
+{
+    __shared__ double smem[LDS_GEMM];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    run_gemm_item(d, (int)(item - d.item_base), smem);
+}
+
+static bool gemm_only_op(int op) { return op == OP_GEMM || op == OP_SYRK; }
+
+// Algorithmic flops of one lowered unit (the whole operation its items cover).
+static double unit_flops(const UnitDesc &u)
+{
+    const double m = u.m, n = u.n, k = u.k;
+    switch (u.op) {
+    case OP_GEMM: return 2.0 * m * n * k;
+    case OP_SYRK: return m * (m + 1) * k;
+    case OP_TRSM: return m * m * n;  // triangle of order m, n right-hand sides
+    case OP_GETRF: return 2.0 * m * m * m / 3.0;
+    case OP_POTRF: return m * m * m / 3.0;
+    default: return 0.0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -112,10 +148,10 @@ __global__ __launch_bounds__(WG_THREADS) void k_persistent(ExecView ev)
         const int u = __builtin_amdgcn_readfirstlane(s_unit);
         if (item < 0)
             break;
-        const UnitDesc *d = ev.units + u;
+        const UnitDesc d = ev.units[u];
         if (ev.trace && tid == 0)
             ev.trace[2 * (size_t)item] = wall_clock64();
-        run_item(*d, item - (int)d->item_base, smem);
+        run_item(d, item - (int)d.item_base, smem);
         // make this item's tile writes visible device-wide before it is
         // counted as complete
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -172,7 +208,10 @@ int run_units(const Lowered &low, hipStream_t s, std::string &err)
     for (const UnitDesc &d : low.units) {
         if (d.nitems <= 0)
             continue;
-        hipLaunchKernelGGL(k_unit, dim3(d.nitems), dim3(WG_THREADS), 0, s, d);
+        if (gemm_only_op(d.op))
+            hipLaunchKernelGGL(k_unit_gemm, dim3(d.nitems), dim3(WG_THREADS), 0, s, d);
+        else
+            hipLaunchKernelGGL(k_unit, dim3(d.nitems), dim3(WG_THREADS), 0, s, d);
         HIPCHK(hipGetLastError());
     }
     return 0;
@@ -249,6 +288,18 @@ int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts
         const UnitDesc *du = reinterpret_cast<const UnitDesc *>(e->dbuf + o_units);
         const uint32_t *diu = reinterpret_cast<const uint32_t *>(e->dbuf + o_iu);
         const uint32_t *ditems = reinterpret_cast<const uint32_t *>(e->dbuf + o_items);
+        e->level_flops.assign(nlev, 0.0);
+        for (uint32_t u = 0; u < nu; u++)
+            if (L.units[u].nitems > 0)
+                e->level_flops[level[u]] += unit_flops(L.units[u]);
+        // a level whose items are all GEMM/SYRK blocks runs on k_batch_gemm
+        e->level_gemm.assign(nlev, 1);
+        for (int l = 0; l < nlev; l++)
+            for (uint32_t it : lv[l])
+                if (!gemm_only_op(L.units[item_unit[it]].op)) {
+                    e->level_gemm[l] = 0;
+                    break;
+                }
         HIPCHK(hipGraphCreate(&e->graph, 0));
         hipGraphNode_t prev = nullptr;
         for (int l = 0; l < nlev; l++) {
@@ -256,7 +307,8 @@ int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts
             void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
             hipKernelNodeParams kp;
             std::memset(&kp, 0, sizeof kp);
-            kp.func = reinterpret_cast<void *>(k_batch);
+            kp.func = e->level_gemm[l] ? reinterpret_cast<void *>(k_batch_gemm)
+                                       : reinterpret_cast<void *>(k_batch);
             kp.gridDim = dim3((unsigned)lv[l].size());
             kp.blockDim = dim3(WG_THREADS);
             kp.kernelParams = args;
@@ -400,7 +452,10 @@ int exec_launch_level(Exec *e, int level, hipStream_t s, std::string &err)
     const uint32_t *diu = (const uint32_t *)e->level_args[1];
     const uint32_t *first = (const uint32_t *)e->level_args[2] + e->level_off[level];
     const unsigned n = (unsigned)(e->level_off[level + 1] - e->level_off[level]);
-    hipLaunchKernelGGL(k_batch, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
+    if (e->level_gemm[level])
+        hipLaunchKernelGGL(k_batch_gemm, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
+    else
+        hipLaunchKernelGGL(k_batch, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -41,6 +41,8 @@ struct Exec {
     int levels = 0;
     std::vector<uint32_t> level_items;
     std::vector<size_t> level_off;
+    std::vector<char> level_gemm;    // 1: the level runs on the GEMM/SYRK kernel
+    std::vector<double> level_flops; // algorithmic flops of the units of each level
     const void *level_args[3] = {nullptr, nullptr, nullptr};
 };
 

@@ -37,9 +37,9 @@ enum Flag : int32_t {
 
 // Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).
 constexpr int WG_THREADS = 256;
-constexpr int GEMM_BM = 128;
-constexpr int GEMM_BN = 128;
-constexpr int GEMM_BK = 16;
+constexpr int GEMM_BM = 64;     // GEMM/SYRK output block rows per item
+constexpr int GEMM_BN = 64;     // ... and columns
+constexpr int GEMM_BK = 8;      // k-block staged in LDS per step
 constexpr int TRSM_W = 16;        // right-hand sides per TRSM item
 constexpr int TRSM_NMAX = 1024;   // largest triangle a TRSM item keeps in LDS
 constexpr int SMALL_MAX = 128;    // largest tile a GETRF/POTRF item factors in LDS
