@@ -198,9 +198,19 @@ def main():
                 p["flops"] += fl
         g = per["k_batch_gemm"]
         achieved = g["flops"] / (g["ms"] * 1e-3) * 1e-12
+        # HBM bytes per k_batch_gemm launch: FETCH_SIZE + WRITE_SIZE from the
+        # rocprofv3 --pmc passes summarised in profiles/ (scaled by the same
+        # counters on a copy of known size); not measurable inside this run
+        traffic = None
+        tp = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tp):
+            with open(tp) as f:
+                tj = json.load(f)
+            if tj.get("kernel") == "k_batch_gemm" and n == 24000 and t == 24:
+                traffic = tj["bytes_per_launch"]
         out["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3),
                            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                           "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                           "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                            "kernel": "k_batch_gemm", "launches": g["launches"],
                            "avg_launch_ms": round(g["ms"] / g["launches"], 4),
                            "avg_launch_gflop": round(g["flops"] / g["launches"] * 1e-9, 3)}

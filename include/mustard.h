@@ -72,7 +72,7 @@ inline void check_hip_(hipError_t e, const char *expr, const char *file, int lin
 
 // Execution modes of an instantiated DAG.
 enum class Mode : int {
-    Tiled = MU_MODE_TILED,       // one hipGraph, a kernel node per unit
+    Tiled = MU_MODE_TILED,       // one hipGraph, one batched kernel node per DAG level
     Subgraph = MU_MODE_SUBGRAPH, // one persistent scheduler kernel
     Stream = MU_MODE_STREAM,     // units launched one by one (debugging)
 };
