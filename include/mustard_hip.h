@@ -142,6 +142,7 @@ int mu_dag_set_tag(mu_dag_t dag, int tag);
 #define MU_OP_TRSM 3
 #define MU_OP_GETRF 4
 #define MU_OP_POTRF 5
+#define MU_OP_TRINV 6 /* internal: inverted 16x16 diagonal blocks of a large TRSM triangle */
 /* Written tile and operation codes of a recorded task (for DOT output and
  * inspection; the reference prints its graphs with cudaGraphDebugDotPrint,
  * lu_mustard.cu:583-585, 634-635). */
@@ -159,7 +160,7 @@ int mu_exec_launch(mu_exec_t exec, void *stream);
  * its item count (per-launch timing; a full run is every level in order). */
 int mu_exec_launch_level(mu_exec_t exec, int level, void *stream);
 int mu_exec_level_items(mu_exec_t exec, int level, long long *items);
-/* Items of one level per operation: counts[MU_OP_*] (6 entries). */
+/* Items of one level per operation: counts[MU_OP_*] (7 entries). */
 int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 /* The kernel one level runs on (MU_KERNEL_GENERIC: k_batch, any item kind;
  * MU_KERNEL_GEMM: k_batch_gemm, GEMM/SYRK blocks only) and the algorithmic

@@ -73,6 +73,16 @@ def dgemm(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, stream=N
          ldb, beta, C, ldc)
 
 
+PREC = {"fp64": 0, "fp32": 1, "bf16": 2}
+
+
+def set_update_precision(prec, stream=None):
+    """Precision of the GEMM/SYRK calls issued next on `stream` ("fp64",
+    "fp32": FP32 operands and accumulation, "bf16": BF16 operands, FP32
+    accumulation); C stays FP64."""
+    call("mu_set_update_precision", stream_handle(stream), PREC[prec] if isinstance(prec, str) else prec)
+
+
 def dsyrk(uplo, trans, n, k, alpha, A, lda, beta, C, ldc, stream=None):
     call("mu_dsyrk", stream_handle(stream), _b(uplo), _b(trans), n, k, alpha, A, lda, beta, C,
          ldc)
@@ -177,10 +187,10 @@ class Exec:
 
     def level_ops(self, level):
         """Items of one level per operation name."""
-        c = (ctypes.c_longlong * 6)()
+        c = (ctypes.c_longlong * 7)()
         call("mu_exec_level_ops", self.h, level, c)
-        names = ["NONE", "GEMM", "SYRK", "TRSM", "GETRF", "POTRF"]
-        return {names[i]: c[i] for i in range(6) if c[i]}
+        names = ["NONE", "GEMM", "SYRK", "TRSM", "GETRF", "POTRF", "TRINV"]
+        return {names[i]: c[i] for i in range(7) if c[i]}
 
     def level_info(self, level):
         """(kernel name, algorithmic flops) of one level of a tiled executable."""

@@ -554,7 +554,7 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts)
     const mu::Exec *e = exec ? exec->e : nullptr;
     if (!e || !counts || level < 0 || level >= e->levels)
         return fail(MU_EINVAL, "mu_exec_level_ops: bad arguments");
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 7; i++)
         counts[i] = 0;
     for (size_t k = e->level_off[level]; k < e->level_off[level + 1]; k++) {
         const uint32_t item = e->level_items[k];
@@ -567,7 +567,7 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts)
             else
                 hi = mid;
         }
-        counts[e->low.units[lo].op % 6]++;
+        counts[e->low.units[lo].op % 7]++;
     }
     return MU_OK;
 }

@@ -84,7 +84,32 @@ int unit_items(const UnitDesc &u)
     case OP_TRSM: return u.gn;
     case OP_GETRF:
     case OP_POTRF: return 1;
+    case OP_TRINV: return u.gm;
     default: return 0;
+    }
+}
+
+size_t ws_total_doubles(const Lowered &low)
+{
+    size_t t = 0;
+    for (size_t v : low.ws_doubles)
+        t += v;
+    return t;
+}
+
+void patch_workspace(std::vector<UnitDesc> &units, const Lowered &low, double *ws)
+{
+    std::vector<size_t> off(low.ws_doubles.size() + 1, 0);
+    for (size_t i = 0; i < low.ws_doubles.size(); i++)
+        off[i + 1] = off[i] + low.ws_doubles[i];
+    for (UnitDesc &d : units) {
+        if (d.ws_slot < 0)
+            continue;
+        double *p = ws + off[d.ws_slot];
+        if (d.op == OP_TRINV)
+            d.c = p;
+        else
+            d.b = p;
     }
 }
 
@@ -114,6 +139,7 @@ struct Builder {
     std::vector<UnitDesc> units;
     std::vector<std::vector<uint32_t>> succ;
     std::vector<int> has_pred;
+    std::vector<size_t> ws_doubles;  // per workspace slot
     int task = 0;
     std::string err;
 
@@ -139,6 +165,7 @@ struct Builder {
     {
         UnitDesc d;
         std::memset(&d, 0, sizeof d);
+        d.ws_slot = -1;
         d.op = op.op;
         d.flags = op.flags;
         d.m = op.m;
@@ -266,6 +293,27 @@ struct Builder {
                 UnitDesc d = base(op);
                 d.gm = 1;
                 d.gn = cdiv(op.n, TRSM_W);
+                if (op.m >= o.trsm_inv_min) {
+                    // the triangle's 16 x 16 diagonal blocks are inverted once,
+                    // by a TRINV unit, for all panels of the solve
+                    const int nblk = cdiv(op.m, 16), slot = (int)ws_doubles.size();
+                    ws_doubles.push_back((size_t)nblk * 256);
+                    UnitDesc t = d;
+                    t.op = OP_TRINV;
+                    t.gm = cdiv(nblk, TRINV_BLOCKS);
+                    t.gn = 1;
+                    t.b = nullptr;
+                    t.c = nullptr;
+                    t.ws_slot = slot;
+                    d.flags |= F_INV;
+                    d.ws_slot = slot;
+                    const int ut = add_unit(t);
+                    const int u = add_unit(d);
+                    edge(ut, u);
+                    f.src = {ut};
+                    f.snk = {u};
+                    return true;
+                }
                 int u = add_unit(d);
                 f.src = {u};
                 f.snk = {u};
@@ -501,6 +549,7 @@ double item_cost_us(const UnitDesc &u)
     case OP_TRSM: return 3.0 + (double)u.m * u.m * TRSM_W / (0.2 * per_cu_flops) + u.m * 0.05;
     case OP_GETRF: return 3.0 + 2.0 * u.m * u.m * (double)u.m / 3.0 / (0.05 * per_cu_flops);
     case OP_POTRF: return 3.0 + u.m * u.m * (double)u.m / 3.0 / (0.05 * per_cu_flops);
+    case OP_TRINV: return 5.0;
     default: return 1.0;
     }
 }
@@ -616,6 +665,7 @@ bool lower(const Dag &dag, const LowerOpts &opts, Lowered &out, std::string &err
     out.nitems = base;
     out.units = std::move(b.units);
     out.succ = std::move(b.succ);
+    out.ws_doubles = std::move(b.ws_doubles);
     return true;
 }
 

@@ -65,6 +65,7 @@ class Dag {
 struct LowerOpts {
     int small_max = SMALL_MAX;  // diagonal factorizations above this nest into sub-tiles
     int trsm_max = TRSM_NMAX;   // triangular solves above this nest into blocks
+    int trsm_inv_min = SMALL_MAX + 1;  // triangles of at least this order use inverted diagonal blocks
     int nprio = NPRIO;
 };
 
@@ -76,7 +77,16 @@ struct Lowered {
     uint64_t nitems = 0;
     double flops = 0.0;  // algorithmic flop count of the recorded ops
     double critical_path = 0.0;
+    // Workspace of the inverted diagonal blocks: slot s holds ws_doubles[s]
+    // doubles; the executor allocates them and points the TRINV unit (c) and
+    // its TRSM unit (b) of each slot there.
+    std::vector<size_t> ws_doubles;
 };
+
+// Point the units of every workspace slot into ws (ws_total_doubles(low)
+// doubles) and return the offsets.
+size_t ws_total_doubles(const Lowered &low);
+void patch_workspace(std::vector<UnitDesc> &units, const Lowered &low, double *ws);
 
 // Lower the recorded DAG to device units (nested sub-DAGs for large
 // factorizations and solves), assign priorities and number the items.

@@ -271,6 +271,208 @@ epilogue:
 }
 
 // ---------------------------------------------------------------------------
+// Reduced-precision GEMM / SYRK block (units flagged F_LOW_F32 or F_LOW_BF16):
+// the FP64 operands are rounded to FP32 (or BF16, round-to-nearest-even) as
+// they are staged into LDS, the products run on FP32-accumulating MFMA
+// (v_mfma_f32_16x16x4f32 / v_mfma_f32_16x16x16bf16_1k), and the result is
+// applied to the FP64 C: C = beta C + alpha (double) acc.  Same 64 x 64 block
+// and 2 x 2 waves as the FP64 kernel; K runs in blocks of 16; both operands
+// are kept k-contiguous in LDS (S[row][k]) so a bf16 lane reads its four k
+// values with one 8-byte load.
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+// Row of element e of a lane's FP32 16 x 16 MFMA result (lane group lk).
+__device__ __forceinline__ int f32_row(int e, int lk) { return 4 * lk + e; }
+
+__device__ __forceinline__ unsigned short to_bf16(double x)
+{
+    // round to FP32, then to BF16 by round-to-nearest-even on the dropped bits
+    unsigned u = __float_as_uint((float)x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <int P>
+struct LowPrec;
+template <>
+struct LowPrec<1> {
+    using T = float;
+    static constexpr int PK = 17;
+    static __device__ __forceinline__ float cv(double x) { return (float)x; }
+};
+template <>
+struct LowPrec<2> {
+    using T = unsigned short;
+    static constexpr int PK = 20;
+    static __device__ __forceinline__ unsigned short cv(double x) { return to_bf16(x); }
+};
+
+template <bool TA, bool TB, bool VEC, int P>
+__device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitDesc &u, int bi, int bj,
+                                                                     bool lower_only, double *smem)
+{
+    using LP = LowPrec<P>;
+    using T = typename LP::T;
+    constexpr int BM = GEMM_BM, BK = 16, PKL = LP::PK, SBUF = BM * PKL;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int li = lane & 15, lk = lane >> 4;
+    const int i0 = bi * BM, j0 = bj * GEMM_BN;
+    const int rows = min(BM, u.m - i0), cols = min(GEMM_BN, u.n - j0);
+    const int K = u.k;
+    const MU_G double *A = TA ? gp(u.a) + (size_t)i0 * u.lda : gp(u.a) + i0;
+    const MU_G double *B = TB ? gp(u.b) + j0 : gp(u.b) + (size_t)j0 * u.ldb;
+    T *S = reinterpret_cast<T *>(smem);
+    T *As[2] = {S, S + 2 * SBUF};
+    T *Bs[2] = {S + SBUF, S + 3 * SBUF};
+
+    // One operand block (64 rows x 16 k) per thread: two element pairs.
+    // i-contiguous storage (element (r,k) at p[r + k*ld]): pair = rows r, r+1.
+    // k-contiguous storage (element (r,k) at p[k + r*ld]): pair = k, k+1.
+    auto ld_op = [&](double (&v)[4], const MU_G double *p, int ld, int nrow, int kc, bool icontig) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int idx = tid + WG_THREADS * q;
+            if (icontig) {
+                const int r = (idx & 31) * 2, c = idx >> 5;
+                const int cc = max(0, min(c, kc - 1));
+                if (VEC) {
+                    const v2d x = *reinterpret_cast<const MU_G v2d *>(p + max(0, min(r, nrow - 2)) +
+                                                                    (size_t)cc * ld);
+                    v[2 * q] = x[0];
+                    v[2 * q + 1] = x[1];
+                } else {
+                    v[2 * q] = p[min(r, nrow - 1) + (size_t)cc * ld];
+                    v[2 * q + 1] = p[min(r + 1, nrow - 1) + (size_t)cc * ld];
+                }
+            } else {
+                const int k = (idx & 7) * 2, r = idx >> 3;
+                const int rr = min(r, nrow - 1);
+                if (VEC) {
+                    const v2d x = *reinterpret_cast<const MU_G v2d *>(p + max(0, min(k, kc - 2)) +
+                                                                    (size_t)rr * ld);
+                    v[2 * q] = x[0];
+                    v[2 * q + 1] = x[1];
+                } else {
+                    v[2 * q] = p[max(0, min(k, kc - 1)) + (size_t)rr * ld];
+                    v[2 * q + 1] = p[max(0, min(k + 1, kc - 1)) + (size_t)rr * ld];
+                }
+            }
+        }
+    };
+    auto st_op = [&](T *D, const double (&v)[4], int nrow, int kc, bool icontig) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int idx = tid + WG_THREADS * q;
+            if (icontig) {
+                const int r = (idx & 31) * 2, c = idx >> 5;
+                const bool cin = c < kc;
+                D[r * PKL + c] = LP::cv((cin && r < nrow) ? v[2 * q] : 0.0);
+                D[(r + 1) * PKL + c] = LP::cv((cin && r + 1 < nrow) ? v[2 * q + 1] : 0.0);
+            } else {
+                const int k = (idx & 7) * 2, r = idx >> 3;
+                const bool rin = r < nrow;
+                D[r * PKL + k] = LP::cv((rin && k < kc) ? v[2 * q] : 0.0);
+                D[r * PKL + k + 1] = LP::cv((rin && k + 1 < kc) ? v[2 * q + 1] : 0.0);
+            }
+        }
+    };
+    double ra0[4], rb0[4], ra1[4], rb1[4];
+    auto load = [&](double (&ra)[4], double (&rb)[4], int k0) {
+        const int kc = K - k0, ks = kc > 0 ? k0 : 0;
+        if (TA)
+            ld_op(ra, A + ks, u.lda, rows, kc, false);
+        else
+            ld_op(ra, A + (size_t)ks * u.lda, u.lda, rows, kc, true);
+        if (TB)
+            ld_op(rb, B + (size_t)ks * u.ldb, u.ldb, cols, kc, true);
+        else
+            ld_op(rb, B + ks, u.ldb, cols, kc, false);
+    };
+    auto store = [&](const double (&ra)[4], const double (&rb)[4], int buf, int k0) {
+        const int kc = K - k0;
+        st_op(As[buf], ra, rows, kc, !TA);
+        st_op(Bs[buf], rb, cols, kc, TB);
+    };
+    v4f acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+            acc[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+    auto compute = [&](int buf) {
+        const T *Sa = As[buf], *Sb = Bs[buf];
+        if (P == 1) {
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                float fa[2], fb[2];
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+                    fa[x] = Sa[(wm * 32 + x * 16 + li) * PKL + kk + lk];
+#pragma unroll
+                for (int y = 0; y < 2; y++)
+                    fb[y] = Sb[(wn * 32 + y * 16 + li) * PKL + kk + lk];
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[x], fb[y], acc[x][y], 0,
+                                                                         0, 0);
+            }
+        } else {
+            v4s fa[2], fb[2];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+                fa[x] = *reinterpret_cast<const v4s *>(Sa + (wm * 32 + x * 16 + li) * PKL + 4 * lk);
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+                fb[y] = *reinterpret_cast<const v4s *>(Sb + (wn * 32 + y * 16 + li) * PKL + 4 * lk);
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[x], fb[y], acc[x][y], 0,
+                                                                          0, 0);
+        }
+    };
+
+    const int nk = ((K + BK - 1) / BK + 1) & ~1;
+    if (K > 0) {
+        load(ra0, rb0, 0);
+        store(ra0, rb0, 0, 0);
+        load(ra1, rb1, BK);
+        lds_barrier();
+        for (int kb = 0; kb < nk; kb += 2) {
+            load(ra0, rb0, (kb + 2) * BK);
+            compute(0);
+            store(ra1, rb1, 1, (kb + 1) * BK);
+            lds_barrier();
+            load(ra1, rb1, (kb + 3) * BK);
+            compute(1);
+            store(ra0, rb0, 0, (kb + 2) * BK);
+            lds_barrier();
+        }
+    }
+    const double alpha = u.alpha, beta = u.beta;
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = wm * 32 + x * 16 + f32_row(e, lk);
+                const int c = wn * 32 + y * 16 + li;
+                if (r < rows && c < cols && (!lower_only || i0 + r >= j0 + c)) {
+                    MU_G double *cp = gp(u.c) + (i0 + r) + (size_t)(j0 + c) * u.ldc;
+                    const double old = beta != 0.0 ? *cp : 0.0;
+                    *cp = beta * old + alpha * (double)acc[x][y][e];
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
 // Panel triangular solve.  Solves T Y = Z in place for one panel of up to 16
 // right-hand sides, T lower triangular of order n <= TRSM_NMAX:
 //   left-lower-unit   X = L^-1 B  : T = L,   Z = B (columns)
@@ -419,6 +621,175 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
         }
         if (r < n && c < w)
             *zp(r, c) = Y[r * 16 + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inverted diagonal blocks.  For a triangle T of order n > SMALL_MAX (read in
+// the canonical lower form the panel solve uses: flags TT, UNIT, REV as in
+// trsm_panel), item `it` inverts diagonal blocks 16*it .. 16*it+15 of 16 x 16
+// and stores inv(T_jj) row-major at c + 256 j.  Rows and columns of the last
+// block past n are the identity.  Thread t: block t/16, column t%16, by
+// forward substitution against the unit vector (divisions, as LAPACK's
+// dtrti2 does; this runs once per triangle).
+// ---------------------------------------------------------------------------
+template <bool TT, bool UNIT, bool REV>
+__device__ __attribute__((always_inline)) inline void trinv_blocks(const UnitDesc &u, int it,
+                                                                   double *smem)
+{
+    const int n = u.m, tid = threadIdx.x, lb = tid >> 4, c = tid & 15;
+    const int j = it * TRINV_BLOCKS + lb, j0 = 16 * j;
+    const MU_G double *t = gp(u.a);
+    const size_t ldt = u.lda;
+    auto tv = [&](int r, int q) -> double {
+        if (REV) {
+            r = n - 1 - r;
+            q = n - 1 - q;
+        }
+        return TT ? t[q + r * ldt] : t[r + q * ldt];
+    };
+    double *D = smem + lb * 16 * 17;
+    const bool live = j0 < n;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const bool in = live && j0 + r < n && j0 + c < n;
+        double v;
+        if (r == c)
+            v = (UNIT || !in) ? 1.0 : tv(j0 + r, j0 + c);
+        else
+            v = (r > c && in) ? tv(j0 + r, j0 + c) : 0.0;
+        D[r * 17 + c] = v;
+    }
+    lds_barrier();
+    double x[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        double s = r == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < r; q++)
+            s -= D[r * 17 + q] * x[q];
+        x[r] = UNIT ? s : s / D[r * 17 + r];
+    }
+    if (live) {
+        MU_G double *w = gp(u.c) + (size_t)j * 256;
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            w[r * 16 + c] = x[r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Panel triangular solve with inverted diagonal blocks (triangles of order
+// n > SMALL_MAX; the TRINV unit before it left inv(T_jj) at u.b).  Same
+// problem and flags as trsm_panel.  Step jb: every wave forms the solved
+// block X_jb = inv(T_jj) Y_jb with four MFMAs -- the result's register layout
+// is exactly the B operand of the update MFMAs, so it never goes back
+// through LDS -- wave 0 stores it to Z, and each wave updates its row blocks
+// below, Y[rb] -= T[rb, jb] X_jb.  One barrier per step.  The next step's
+// inverse block and triangle fragments are loaded during the current step
+// (the latter row block by row block, right after that block's MFMAs).
+// ---------------------------------------------------------------------------
+template <bool TT, bool ZR, bool UNIT, bool REV>
+__device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitDesc &u, int panel,
+                                                                     double *smem)
+{
+    const int n = u.m, p0 = panel * TRSM_W, w = min(TRSM_W, u.n - p0);
+    const int np = (n + 15) & ~15, nblk = np >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    double *Y = smem;  // [np][16]
+    const MU_G double *t = gp(u.a);
+    const MU_G double *winv = gp(u.b);
+    const size_t ldt = u.lda, ldz = u.ldc;
+    MU_G double *z = gp(u.c);
+    const double alpha = u.alpha;
+    auto tv = [&](int r, int c) -> double {
+        if (REV) {
+            r = n - 1 - r;
+            c = n - 1 - c;
+        }
+        return TT ? t[c + r * ldt] : t[r + c * ldt];
+    };
+    auto zp = [&](int r, int c) -> MU_G double * {
+        if (REV)
+            r = n - 1 - r;
+        return ZR ? z + (p0 + c) + r * ldz : z + r + (p0 + c) * ldz;
+    };
+
+    for (int idx = tid; idx < np * 16; idx += WG_THREADS) {
+        int r, c;
+        if (ZR) {
+            c = idx & 15;
+            r = idx >> 4;
+        } else {
+            r = idx % np;
+            c = idx / np;
+        }
+        Y[r * 16 + c] = (r < n && c < w) ? alpha * *zp(r, c) : 0.0;
+    }
+    lds_barrier();
+
+    constexpr int MAXB = TRSM_NMAX / 64;
+    auto load_inv = [&](double (&wv)[4], int jb) {
+        const MU_G double *b = winv + (size_t)min(jb >> 4, nblk - 1) * 256 + li * 16 + lk;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+            wv[kk] = b[4 * kk];
+    };
+    auto load_block = [&](double (&f)[4], int jb, int i) {
+        const int rb = jb + 16 + 16 * wave + 64 * i;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+            f[kk] = tv(min(rb + li, n - 1), min(jb + 4 * kk + lk, n - 1));
+    };
+    auto step = [&](int jb, const double (&wv)[4], const double (&fa)[MAXB][4], double (&wn)[4],
+                    double (&fn)[MAXB][4]) {
+        double yb[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+            yb[kk] = Y[(jb + 4 * kk + lk) * 16 + li];
+        v4d xs = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++)
+            xs = mfma_f64(wv[kk], yb[kk], xs);
+        // xs[e] = X[jb + 4e + lk][li]
+        if (wave == 0) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = jb + 4 * e + lk;
+                if (r < n && li < w)
+                    *zp(r, li) = xs[e];
+            }
+        }
+        load_inv(wn, jb + 16);
+#pragma unroll
+        for (int i = 0; i < MAXB; i++) {
+            const int rb = jb + 16 + 16 * wave + 64 * i;
+            if (rb < np) {
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const bool in = rb + li < n && jb + 4 * kk + lk < n;
+                    acc = mfma_f64(in ? fa[i][kk] : 0.0, xs[kk], acc);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    Y[(rb + 4 * e + lk) * 16 + li] -= acc[e];
+            }
+            load_block(fn[i], jb + 16, i);
+        }
+        lds_barrier();
+    };
+    double fa0[MAXB][4], fa1[MAXB][4], w0[4], w1[4];
+    load_inv(w0, 0);
+#pragma unroll
+    for (int i = 0; i < MAXB; i++)
+        load_block(fa0[i], 0, i);
+    for (int jb = 0; jb < n; jb += 32) {
+        step(jb, w0, fa0, w1, fa1);
+        if (jb + 16 >= n)
+            break;
+        step(jb + 16, w1, fa1, w0, fa0);
     }
 }
 
@@ -590,6 +961,10 @@ __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc
 constexpr int LDS_GEMM = 2 * (LDS_I > LDS_K ? LDS_I : LDS_K) * 2;  // 2 operands x 2 buffers
 constexpr int LDS_TRSM = TRSM_NMAX * 16 + 16 * 17;
 constexpr int LDS_SMALL = 516;
+static_assert(TRINV_BLOCKS * 16 * 17 <= LDS_TRSM, "TRINV staging fits the generic LDS");
+static_assert(TRINV_BLOCKS * 16 == WG_THREADS, "TRINV: one thread per block column");
+static_assert(4 * GEMM_BM * 20 * 2 <= LDS_GEMM * 8 && 4 * GEMM_BM * 17 * 4 <= LDS_GEMM * 8,
+              "reduced-precision GEMM buffers fit the GEMM LDS");
 constexpr int LDS_DOUBLES = LDS_GEMM > LDS_TRSM ? (LDS_GEMM > LDS_SMALL ? LDS_GEMM : LDS_SMALL)
                                                 : (LDS_TRSM > LDS_SMALL ? LDS_TRSM : LDS_SMALL);
 
@@ -609,10 +984,21 @@ template <bool TA, bool TB>
 __device__ __forceinline__ void gemm_dispatch_vec(const UnitDesc &u, int bi, int bj, bool lo,
                                                   double *smem)
 {
-    if (u.flags & F_VEC)
+    if (u.flags & F_LOW_F32) {
+        if (u.flags & F_VEC)
+            gemm_block_low<TA, TB, true, 1>(u, bi, bj, lo, smem);
+        else
+            gemm_block_low<TA, TB, false, 1>(u, bi, bj, lo, smem);
+    } else if (u.flags & F_LOW_BF16) {
+        if (u.flags & F_VEC)
+            gemm_block_low<TA, TB, true, 2>(u, bi, bj, lo, smem);
+        else
+            gemm_block_low<TA, TB, false, 2>(u, bi, bj, lo, smem);
+    } else if (u.flags & F_VEC) {
         gemm_block<TA, TB, true>(u, bi, bj, lo, smem);
-    else
+    } else {
         gemm_block<TA, TB, false>(u, bi, bj, lo, smem);
+    }
 }
 
 // GEMM or SYRK item `it` of unit `u` (the only kinds k_batch_gemm runs).
@@ -643,11 +1029,15 @@ __device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u
     case OP_GEMM:
     case OP_SYRK: run_gemm_item(u, it, smem); break;
     case OP_TRSM:
-        switch ((u.flags >> 3) & 0x27) {  // T_TRANS, Z_ROWS, UNIT, REV
-#define MU_TRSM_CASE(bits)                                                                  \
-    case bits:                                                                              \
-        trsm_panel<(bits & 1) != 0, (bits & 2) != 0, (bits & 4) != 0, (bits & 0x20) != 0>( \
-            u, it, smem);                                                                   \
+        switch ((u.flags >> 3) & 0x67) {  // T_TRANS, Z_ROWS, UNIT, REV, INV
+#define MU_TRSM_CASE(bits)                                                                      \
+    case bits:                                                                                  \
+        trsm_panel<(bits & 1) != 0, (bits & 2) != 0, (bits & 4) != 0, (bits & 0x20) != 0>(     \
+            u, it, smem);                                                                       \
+        break;                                                                                  \
+    case bits | 0x40:                                                                           \
+        trsm_panel_inv<(bits & 1) != 0, (bits & 2) != 0, (bits & 4) != 0, (bits & 0x20) != 0>( \
+            u, it, smem);                                                                       \
         break;
             MU_TRSM_CASE(0x00) MU_TRSM_CASE(0x01) MU_TRSM_CASE(0x02) MU_TRSM_CASE(0x03)
             MU_TRSM_CASE(0x04) MU_TRSM_CASE(0x05) MU_TRSM_CASE(0x06) MU_TRSM_CASE(0x07)
@@ -659,6 +1049,18 @@ __device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u
         break;
     case OP_GETRF: getrf_small(u, smem); break;
     case OP_POTRF: potrf_small(u, smem); break;
+    case OP_TRINV:
+        switch ((u.flags >> 3) & 0x25) {  // T_TRANS, UNIT, REV
+        case 0x00: trinv_blocks<false, false, false>(u, it, smem); break;
+        case 0x01: trinv_blocks<true, false, false>(u, it, smem); break;
+        case 0x04: trinv_blocks<false, true, false>(u, it, smem); break;
+        case 0x05: trinv_blocks<true, true, false>(u, it, smem); break;
+        case 0x20: trinv_blocks<false, false, true>(u, it, smem); break;
+        case 0x21: trinv_blocks<true, false, true>(u, it, smem); break;
+        case 0x24: trinv_blocks<false, true, true>(u, it, smem); break;
+        default: trinv_blocks<true, true, true>(u, it, smem); break;
+        }
+        break;
     default: break;
     }
 }

@@ -205,6 +205,27 @@ static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 int run_units(const Lowered &low, hipStream_t s, std::string &err)
 {
+    // units of an executable come patched; an eager call gets a
+    // stream-ordered workspace for the life of its launches
+    const size_t wsd = ws_total_doubles(low);
+    bool patched = true;
+    for (const UnitDesc &d : low.units)
+        if (d.ws_slot >= 0 && ((d.op == OP_TRINV && !d.c) || (d.op == OP_TRSM && !d.b)))
+            patched = false;
+    if (wsd && !patched) {
+        double *ws = nullptr;
+        HIPCHK(hipMallocAsync((void **)&ws, wsd * sizeof(double), s));
+        std::vector<UnitDesc> units = low.units;
+        patch_workspace(units, low, ws);
+        Lowered copy;
+        copy.units = std::move(units);
+        int rc = run_units(copy, s, err);
+        hipError_t fe = hipFreeAsync(ws, s);
+        if (rc != 0)
+            return rc;
+        HIPCHK(fe);
+        return 0;
+    }
     for (const UnitDesc &d : low.units) {
         if (d.nitems <= 0)
             continue;
@@ -217,18 +238,33 @@ int run_units(const Lowered &low, hipStream_t s, std::string &err)
     return 0;
 }
 
+static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
+                      const ExecOpts &eopts, Exec **out, std::string &err);
+
 int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts &eopts,
                 Exec **out, std::string &err)
 {
     Exec *e = new Exec();
     e->mode = mode;
-    if (!lower(dag, lopts, e->low, err)) {
-        delete e;
+    if (exec_build(e, dag, mode, lopts, eopts, out, err) != 0) {
+        exec_destroy(e);  // frees whatever the failed build had allocated
         return -1;
     }
+    return 0;
+}
+
+static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
+                      const ExecOpts &eopts, Exec **out, std::string &err)
+{
+    if (!lower(dag, lopts, e->low, err))
+        return -1;
     Lowered &L = e->low;
     const uint32_t nu = (uint32_t)L.units.size();
     HIPCHK(hipGetDevice(&e->device));
+    if (const size_t wsd = ws_total_doubles(L)) {
+        HIPCHK(hipMalloc((void **)&e->ws, wsd * sizeof(double)));
+        patch_workspace(L.units, L, e->ws);
+    }
 
     // CSR successors and per-priority ring capacities
     std::vector<uint32_t> succ;
@@ -496,6 +532,8 @@ void exec_destroy(Exec *e)
         (void)hipGraphDestroy(e->graph);
     if (e->dbuf)
         (void)hipFree(e->dbuf);
+    if (e->ws)
+        (void)hipFree(e->ws);
     delete e;
 }
 

@@ -28,6 +28,7 @@ struct Exec {
     // device memory
     char *dbuf = nullptr;
     size_t dbytes = 0;
+    double *ws = nullptr;            // inverted diagonal blocks of the TRSM units
     ExecView ev{};
     uint32_t *slots_init = nullptr;  // initial queue contents, same layout as the rings
     uint32_t slots_total = 0;

@@ -20,6 +20,8 @@ enum Op : int32_t {
     OP_TRSM = 3,   // panel triangular solve T Y = Z; items = panels of <= 16 right-hand sides
     OP_GETRF = 4,  // no-pivot LU of an n <= SMALL_MAX tile in LDS; one item
     OP_POTRF = 5,  // lower Cholesky of an n <= SMALL_MAX tile in LDS; one item
+    OP_TRINV = 6,  // inverses of the 16 x 16 diagonal blocks of a TRSM triangle, into the
+                   // unit's workspace slot; items = groups of TRINV_BLOCKS blocks
 };
 
 // Unit flags.
@@ -33,6 +35,7 @@ enum Flag : int32_t {
     F_LOW_F32 = 1 << 6,  // GEMM/SYRK: operands rounded to FP32, FP32 MFMA, FP32 accumulation
     F_LOW_BF16 = 1 << 7, // GEMM/SYRK: operands rounded to BF16, FP32 accumulation
     F_REV = 1 << 8,      // TRSM: row order of T and Z reversed (upper-triangular solves)
+    F_INV = 1 << 9,      // TRSM: diagonal blocks solved with the inverses a TRINV unit left in b
 };
 
 // Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).
@@ -44,6 +47,7 @@ constexpr int TRSM_W = 16;        // right-hand sides per TRSM item
 constexpr int TRSM_NMAX = 1024;   // largest triangle a TRSM item keeps in LDS
 constexpr int SMALL_MAX = 128;    // largest tile a GETRF/POTRF item factors in LDS
 constexpr int NPRIO = 4;          // priority levels of the device queue
+constexpr int TRINV_BLOCKS = 16;  // diagonal blocks one TRINV item inverts (16 threads each)
 
 struct UnitDesc {
     int32_t op;
@@ -64,7 +68,8 @@ struct UnitDesc {
     const double *b;
     double *c;
     int32_t *info;         // LAPACK-style info cell (atomic min of failing column), or null
-    int32_t pad_[2];
+    int32_t ws_slot;       // TRSM with F_INV / TRINV: workspace slot of the inverted blocks, or -1
+    int32_t pad_;
 };
 static_assert(sizeof(UnitDesc) == 128, "UnitDesc must stay 128 bytes");
 

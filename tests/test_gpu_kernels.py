@@ -71,6 +71,65 @@ def test_dgemm(torch_gpu, m, n, k, ta, tb, pad, alpha, beta):
         assert np.array_equal(got[m:, :], C[m:, :])
 
 
+LOW_CASES = [(1, 1, 1, "N", "N", 0), (17, 5, 3, "N", "T", 0), (64, 64, 16, "T", "N", 0),
+             (129, 130, 131, "T", "T", 1), (300, 257, 1000, "N", "N", 3), (1000, 1000, 1000, "N", "T", 24)]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("m,n,k,ta,tb,pad", LOW_CASES)
+def test_dgemm_reduced_precision(torch_gpu, prec, m, n, k, ta, tb, pad):
+    """FP32 / BF16 trailing-update GEMM: operands rounded, FP32 accumulation,
+    FP64 C.  Bound per element: operand rounding (2u per product) plus an
+    FP32 length-k sum, times (|A||B|)_ij."""
+    mu = _mu()
+    rng = np.random.default_rng(m * 5 + n * 3 + k)
+    ar, ac = (k, m) if ta == "T" else (m, k)
+    br, bc = (n, k) if tb == "T" else (k, n)
+    lda, ldb, ldc = ar + pad, br + pad, m + pad
+    A, B, C = rand(rng, lda, ac), rand(rng, ldb, bc), rand(rng, ldc, n)
+    dA, dB, dC = _dev(A), _dev(B), _dev(C)
+    mu.set_update_precision(prec)
+    try:
+        mu.dgemm(ta, tb, m, n, k, -1.0, mu.ptr(dA), lda, mu.ptr(dB), ldb, 1.0, mu.ptr(dC), ldc)
+    finally:
+        mu.set_update_precision("fp64")
+    torch_gpu.cuda.synchronize()
+    opA = A[:ar, :ac].T if ta == "T" else A[:ar, :ac]
+    opB = B[:br, :bc].T if tb == "T" else B[:br, :bc]
+    ref = C[:m, :n] - opA @ opB
+    got = mu.to_host(dC)
+    u_op = 2.0 ** -24 if prec == "fp32" else 2.0 ** -8
+    tol = (2.5 * u_op + (k + 2) * 2.0 ** -24) * (np.abs(opA) @ np.abs(opB)) + 2 * EPS * np.abs(C[:m, :n])
+    assert np.all(np.abs(got[:m, :n] - ref) <= tol)
+    if prec == "fp32" and k >= 64:
+        # and it really is reduced: not FP64 accurate
+        assert np.max(np.abs(got[:m, :n] - ref)) > 1e3 * EPS
+    if pad:
+        assert np.array_equal(got[m:, :], C[m:, :])
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_dsyrk_reduced_precision(torch_gpu, prec):
+    mu = _mu()
+    n, k = 300, 1000
+    rng = np.random.default_rng(7)
+    A, C = rand(rng, n, k), rand(rng, n, n)
+    dA, dC = _dev(A), _dev(C)
+    mu.set_update_precision(prec)
+    try:
+        mu.dsyrk("L", "N", n, k, -1.0, mu.ptr(dA), n, 1.0, mu.ptr(dC), n)
+    finally:
+        mu.set_update_precision("fp64")
+    torch_gpu.cuda.synchronize()
+    got = mu.to_host(dC)
+    ref = C - A @ A.T
+    u_op = 2.0 ** -24 if prec == "fp32" else 2.0 ** -8
+    tol = (2.5 * u_op + (k + 2) * 2.0 ** -24) * (np.abs(A) @ np.abs(A.T)) + 2 * EPS * np.abs(C)
+    lo = np.tril(np.ones((n, n), dtype=bool))
+    assert np.all(np.abs(got - ref)[lo] <= tol[lo])
+    assert np.array_equal(got[~lo], C[~lo])
+
+
 @pytest.mark.parametrize("n,k,pad", [(1, 1, 0), (17, 3, 0), (128, 64, 0), (129, 129, 1),
                                      (300, 1000, 2), (1000, 1000, 24)])
 def test_dsyrk_lower(torch_gpu, n, k, pad):
@@ -101,8 +160,10 @@ def _tri(rng, n, lower, unit):
     return np.asfortranarray(T)
 
 
-TRSM_SIZES = [(1, 1), (16, 16), (17, 5), (100, 37), (37, 100), (1000, 64), (64, 1000),
-              (1100, 20), (20, 1100)]
+# Triangles of order <= 128 take the serial diagonal path; larger ones go
+# through a TRINV unit (inverted 16 x 16 diagonal blocks); above 1024 they nest.
+TRSM_SIZES = [(1, 1), (16, 16), (17, 5), (100, 37), (37, 100), (128, 9), (129, 7), (7, 129),
+              (300, 50), (50, 300), (1000, 64), (64, 1000), (1100, 20), (20, 1100)]
 
 
 @pytest.mark.parametrize("side", ["L", "R"])
