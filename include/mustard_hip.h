@@ -186,6 +186,61 @@ int mu_exec_destroy(mu_exec_t exec);
 int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info_dev, int prec);
 int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int *info_dev,
                              int prec);
+/* Triangular solves with the factors of a tiled factorization, recorded on
+ * the same kind of DAG (forward then back substitution, one task per tile
+ * operation on the right-hand-side block rows): F holds the factors left by
+ * mu_record_tiled_lu (kind MU_SOLVE_LU: L unit lower, U upper) or
+ * mu_record_tiled_cholesky (MU_SOLVE_CHOL: L lower); X (n x nrhs, ldx) holds
+ * B and is overwritten with the solution.  The reference has no solve phase
+ * (it stops after the factorization, lu_mustard.cu:650-660); this is the
+ * GETRS/POTRS step the refinement below is built on. */
+#define MU_SOLVE_LU 0
+#define MU_SOLVE_CHOL 1
+int mu_record_tiled_solve(mu_dag_t dag, const double *F, int n, int t, int ldf, int kind,
+                          double *X, int ldx, int nrhs);
+
+/* ---- Mixed precision + FP64 iterative refinement ------------------------
+ * No counterpart in the reference: the precision form of the tiled
+ * factorizations.  Solves A X = B (A n x n, B and X n x nrhs, device
+ * pointers; A and B are not modified) by a tiled factorization whose trailing
+ * GEMM/SYRK run in `prec` (diagonal factorizations, panel solves, triangular
+ * solves, residual and correction in FP64), followed by the refinement of
+ * LAPACK dsgesv/dsposv: x = x + solve(b - A x) until
+ * ||r_j||_inf <= ||x_j||_inf ||A||_inf 2^-53 sqrt(n) for every column, at
+ * most itermax times (0: 30); with `fallback`, a run that does not converge
+ * is refactored and solved in FP64.  The factors live in a library-owned
+ * n x n workspace.  Graph construction happens before the timed region. */
+typedef struct {
+    int kind;     /* MU_SOLVE_LU or MU_SOLVE_CHOL */
+    int tiles;    /* tiles per dimension (n % tiles == 0) */
+    int prec;     /* MU_PREC_FP64 / MU_PREC_FP32 / MU_PREC_BF16 trailing updates */
+    int itermax;  /* refinement iterations cap; 0 = 30 */
+    int fallback; /* 1: refactor in FP64 when refinement does not converge */
+} mu_ir_opts_t;
+typedef struct {
+    int info;            /* factorization info (0 = fine) */
+    int iters;           /* refinement steps taken */
+    int converged;       /* the stopping test held */
+    int fell_back;       /* the FP64 fallback ran */
+    double anorm;        /* ||A||_inf */
+    double berr0;        /* normwise backward error of the unrefined solution */
+    double berr;         /* final max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) */
+    double berr_hist[32];/* backward error after each step */
+    double fact_ms;      /* device time: ||A||, copy and factorization */
+    double solve_ms;     /* device time: solves, residuals, corrections */
+    double total_ms;     /* both */
+} mu_ir_result_t;
+int mu_solve_ir(const double *A, int n, int lda, const double *B, int ldb, double *X, int ldx,
+                int nrhs, const mu_ir_opts_t *opts, mu_ir_result_t *res, void *stream);
+
+/* Test matrices of a given condition number (the mixed-precision inputs):
+ * A = Q diag(s) Q^T with s_i = kappa^(-i/(n-1)) (so ||A||_2 = 1 and
+ * cond_2(A) = kappa, A symmetric positive definite) and Q the product of
+ * `nrefl` Householder reflectors I - 2 v v^T / v^T v, v uniform [-1, 1) from
+ * the counter-based stream of mu_gen_uniform (reflector r uses stream seed+r).
+ * work: 3n + 2 doubles of device memory. */
+int mu_gen_cond(double *A, int n, int lda, double kappa, int nrefl, uint64_t seed, double *work,
+                void *stream);
 
 /* ---- generators (include/gen.h) ----------------------------------------- */
 /* gen.h:23-38: glibc srand(420) symmetric matrix plus n I, host memory. */

@@ -130,6 +130,13 @@ class Dag:
         call("mu_record_tiled_cholesky", self.h, A, n, t, lda or n, info, prec)
         return self
 
+    def record_solve(self, F, n, t, kind, X, nrhs, ldf=None, ldx=None):
+        """Forward + back substitution with the factors F of a tiled LU
+        (kind "lu") or Cholesky ("chol") on the right-hand sides X, in place."""
+        call("mu_record_tiled_solve", self.h, F, n, t, ldf or n,
+             _lib.SOLVE_LU if kind == "lu" else _lib.SOLVE_CHOL, X, ldx or n, nrhs)
+        return self
+
     def set_tag(self, tag):
         call("mu_dag_set_tag", self.h, tag)
 
@@ -275,6 +282,34 @@ def scaled_residual(A, F, n, chol=False, lda=None, stream=None):
          ctypes.byref(out), stream_handle(stream))
     del work
     return out.value
+
+
+def gen_cond(n, kappa, seed, nrefl=4, device="cuda", stream=None):
+    """A = Q diag(kappa^(-i/(n-1))) Q^T on the device, Q a product of `nrefl`
+    Householder reflectors: symmetric positive definite, cond_2(A) = kappa."""
+    import torch
+    A = torch.empty((n, n), dtype=torch.float64, device=device)
+    work = torch.empty(3 * n + 2, dtype=torch.float64, device=device)
+    call("mu_gen_cond", ptr(A), n, n, float(kappa), nrefl, seed, ptr(work), stream_handle(stream))
+    torch.cuda.synchronize()
+    del work
+    return A
+
+
+def solve_ir(kind, A, B, n, nrhs, tiles, prec="fp32", itermax=0, fallback=False, stream=None):
+    """X solving A X = B (column-major device matrices as from to_device) by a
+    tiled factorization with `prec` trailing updates and FP64 iterative
+    refinement.  Returns (X, result dict)."""
+    import torch
+    X = torch.zeros((nrhs, n), dtype=torch.float64, device=A.device)
+    o = _lib.IROpts(_lib.SOLVE_LU if kind == "lu" else _lib.SOLVE_CHOL, tiles,
+                    PREC[prec] if isinstance(prec, str) else prec, itermax, 1 if fallback else 0)
+    r = _lib.IRResult()
+    call("mu_solve_ir", ptr(A), n, n, ptr(B), n, ptr(X), n, nrhs, ctypes.byref(o),
+         ctypes.byref(r), stream_handle(stream))
+    res = {f: getattr(r, f) for f, _ in _lib.IRResult._fields_ if f != "berr_hist"}
+    res["berr_hist"] = list(r.berr_hist[:min(32, r.iters + 1)])
+    return X, res
 
 
 def gen_spd(n, seed, device="cuda", stream=None):

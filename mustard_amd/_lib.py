@@ -37,6 +37,21 @@ class ExecStats(ctypes.Structure):
                 ("levels", ctypes.c_int)]
 
 
+class IROpts(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("tiles", ctypes.c_int), ("prec", ctypes.c_int),
+                ("itermax", ctypes.c_int), ("fallback", ctypes.c_int)]
+
+
+class IRResult(ctypes.Structure):
+    _fields_ = [("info", ctypes.c_int), ("iters", ctypes.c_int), ("converged", ctypes.c_int),
+                ("fell_back", ctypes.c_int), ("anorm", ctypes.c_double),
+                ("berr0", ctypes.c_double), ("berr", ctypes.c_double),
+                ("berr_hist", ctypes.c_double * 32), ("fact_ms", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+SOLVE_LU, SOLVE_CHOL = 0, 1
+
 _vp, _i, _d, _c = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_char
 _sz, _u64 = ctypes.c_size_t, ctypes.c_uint64
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -96,6 +111,10 @@ SIGNATURES = {
     "mu_gen_ref_host": (None, [_vp, _sz]),
     "mu_gen_uniform": (_i, [_vp, _sz, _u64, _d, _d, _vp]),
     "mu_gen_spd": (_i, [_vp, _i, _i, _u64, _vp, _vp]),
+    "mu_gen_cond": (_i, [_vp, _i, _i, _d, _i, _u64, _vp, _vp]),
+    "mu_record_tiled_solve": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _i, _i]),
+    "mu_solve_ir": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _i, ctypes.POINTER(IROpts),
+                         ctypes.POINTER(IRResult), _vp]),
     "mu_verify_lu_l1_host": (_d, [_vp, _vp, _i]),
     "mu_verify_chol_l1_host": (_d, [_vp, _vp, _i]),
     "mu_scaled_residual": (_i, [_vp, _vp, _i, _i, _i, _vp, ctypes.POINTER(_d), _vp]),

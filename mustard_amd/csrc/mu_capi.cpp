@@ -34,6 +34,10 @@ struct mu_dag_s {
     mu::Dag dag;
 };
 
+namespace mu {
+int set_error(int code, const std::string &msg);
+}
+
 struct mu_exec_s {
     mu::Exec *e = nullptr;
 };
@@ -47,6 +51,12 @@ int fail(int code, const std::string &msg)
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+int mu::set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
 
 #define HIPTRY(x)                                                                      \
     do {                                                                               \
@@ -799,6 +809,72 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
     return rc ? rc : rc2;
 }
 
+int mu_record_tiled_solve(mu_dag_t dag, const double *F, int n, int t, int ldf, int kind,
+                          double *X, int ldx, int nrhs)
+{
+    if (!dag || !F || !X || n <= 0 || t <= 0 || n % t != 0 || ldf < n || ldx < n || nrhs <= 0 ||
+        (kind != MU_SOLVE_LU && kind != MU_SOLVE_CHOL))
+        return fail(MU_EINVAL, "mu_record_tiled_solve: need n % t == 0, ldf, ldx >= n, nrhs > 0, "
+                               "kind MU_SOLVE_LU or MU_SOLVE_CHOL");
+    const int b = n / t;
+    void *key = (char *)dag + 1;  // a recording key distinct from the factorization's
+    int rc;
+    if ((rc = mu_dag_capture_begin(dag, key)) != 0)
+        return rc;
+    double *Fm = const_cast<double *>(F);
+    auto fb = [&](int i, int j) { return Fm + (size_t)i * b + (size_t)j * b * ldf; };
+    auto xb = [&](int i) { return X + (size_t)i * b; };
+    // right-hand-side block i is tile (i, t): it never aliases a factor tile
+    auto task = [&](int w, std::initializer_list<int> reads) {
+        std::vector<int> r(reads);
+        return mu_dag_begin_task(dag, w, t, r.data(), (int)r.size() / 2, nullptr);
+    };
+    // forward substitution: L Y = B (L unit lower for LU, lower for Cholesky)
+    const char fdiag = kind == MU_SOLVE_LU ? 'U' : 'N';
+    for (int k = 0; k < t && rc == 0; k++) {
+        rc = task(k, {k, k, k, t});
+        if (!rc)
+            rc = mu_dtrsm(key, 'L', 'L', 'N', fdiag, b, nrhs, 1.0, fb(k, k), ldf, xb(k), ldx);
+        if (!rc)
+            rc = mu_dag_end_task(dag);
+        for (int i = k + 1; i < t && rc == 0; i++) {
+            rc = task(i, {i, k, k, t, i, t});
+            if (!rc)
+                rc = mu_dgemm(key, 'N', 'N', b, nrhs, b, -1.0, fb(i, k), ldf, xb(k), ldx, 1.0,
+                              xb(i), ldx);
+            if (!rc)
+                rc = mu_dag_end_task(dag);
+        }
+    }
+    // back substitution: U X = Y (LU) or L^T X = Y (Cholesky)
+    for (int k = t - 1; k >= 0 && rc == 0; k--) {
+        rc = task(k, {k, k, k, t});
+        if (!rc)
+            rc = kind == MU_SOLVE_LU
+                     ? mu_dtrsm(key, 'L', 'U', 'N', 'N', b, nrhs, 1.0, fb(k, k), ldf, xb(k), ldx)
+                     : mu_dtrsm(key, 'L', 'L', 'T', 'N', b, nrhs, 1.0, fb(k, k), ldf, xb(k), ldx);
+        if (!rc)
+            rc = mu_dag_end_task(dag);
+        for (int i = 0; i < k && rc == 0; i++) {
+            rc = kind == MU_SOLVE_LU ? task(i, {i, k, k, t, i, t}) : task(i, {k, i, k, t, i, t});
+            if (!rc)
+                rc = kind == MU_SOLVE_LU
+                         ? mu_dgemm(key, 'N', 'N', b, nrhs, b, -1.0, fb(i, k), ldf, xb(k), ldx,
+                                    1.0, xb(i), ldx)
+                         : mu_dgemm(key, 'T', 'N', b, nrhs, b, -1.0, fb(k, i), ldf, xb(k), ldx,
+                                    1.0, xb(i), ldx);
+            if (!rc)
+                rc = mu_dag_end_task(dag);
+        }
+    }
+    int rc2 = mu_dag_capture_end(dag, key);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_streams.erase(key);
+    }
+    return rc ? rc : rc2;
+}
+
 // ---- generators -------------------------------------------------------------
 void mu_gen_ref_host(double *h_A, size_t n)
 {
@@ -836,6 +912,31 @@ int mu_gen_spd(double *A, int n, int lda, uint64_t seed, double *work, void *str
     if (mu::add_diag_scale(A, n, lda, (double)n, (double)n, (hipStream_t)stream) != 0 ||
         mu::symmetrize_lower(A, n, lda, (hipStream_t)stream) != 0)
         return fail(MU_EHIP, "kernel launch failed");
+    return MU_OK;
+}
+
+int mu_gen_cond(double *A, int n, int lda, double kappa, int nrefl, uint64_t seed, double *work,
+                void *stream)
+{
+    if (!A || !work || n <= 0 || lda < n || !(kappa >= 1.0) || nrefl < 0)
+        return fail(MU_EINVAL, "mu_gen_cond: bad arguments (kappa >= 1, nrefl >= 0)");
+    hipStream_t s = (hipStream_t)stream;
+    double *v = work, *w = work + n, *sc = work + 2 * (size_t)n;
+    if (mu::gen_cond_start(A, n, lda, kappa, s) != 0)
+        return fail(MU_EHIP, "kernel launch failed");
+    for (int r = 0; r < nrefl; r++) {
+        int rc = mu_gen_uniform(v, (size_t)n, seed + (uint64_t)r, -1.0, 1.0, stream);
+        if (rc)
+            return rc;
+        // w = A v (the current, symmetric A), sc = (v.v, v.w)
+        rc = mu_dgemm(stream, 'N', 'N', n, 1, n, 1.0, A, lda, v, n, 0.0, w, n);
+        if (rc)
+            return rc;
+        HIPTRY(hipMemsetAsync(sc, 0, 2 * sizeof(double), s));
+        if (mu::dot_acc(v, v, n, sc, s) != 0 || mu::dot_acc(v, w, n, sc + 1, s) != 0 ||
+            mu::reflect2(A, lda, n, v, w, sc, s) != 0)
+            return fail(MU_EHIP, "kernel launch failed");
+    }
     return MU_OK;
 }
 

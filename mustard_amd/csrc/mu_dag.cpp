@@ -289,6 +289,11 @@ struct Builder {
         case OP_TRSM:
             if (op.m <= 0 || op.n <= 0)
                 return true;
+            if ((int64_t)(std::min(op.m, o.trsm_max) + 1) * op.lda >= ((int64_t)1 << 31)) {
+                // the panel kernels address the triangle with 32-bit offsets
+                err = "TRSM: leading dimension of the triangle too large (order x lda >= 2^31)";
+                return false;
+            }
             if (op.m <= o.trsm_max) {
                 UnitDesc d = base(op);
                 d.gm = 1;

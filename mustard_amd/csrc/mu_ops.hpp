@@ -495,12 +495,17 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
     const size_t ldt = u.lda, ldz = u.ldc;
     MU_G double *z = gp(u.c);
     const double alpha = u.alpha;
+    // 32-bit offsets with 24-bit multiplies (full rate, unlike 64-bit
+    // products): the lowering guarantees (n + 1) * ldt < 2^31 for triangles
+    const unsigned ldt32 = (unsigned)ldt;
     auto tv = [&](int r, int c) -> double {
         if (REV) {
             r = n - 1 - r;
             c = n - 1 - c;
         }
-        return TT ? t[c + r * ldt] : t[r + c * ldt];
+        const unsigned off = TT ? (unsigned)c + __umul24((unsigned)r, ldt32)
+                                : (unsigned)r + __umul24((unsigned)c, ldt32);
+        return t[off];
     };
     auto zp = [&](int r, int c) -> MU_G double * {
         if (REV)
@@ -641,12 +646,15 @@ __device__ __attribute__((always_inline)) inline void trinv_blocks(const UnitDes
     const int j = it * TRINV_BLOCKS + lb, j0 = 16 * j;
     const MU_G double *t = gp(u.a);
     const size_t ldt = u.lda;
+    const unsigned ldt32 = (unsigned)ldt;
     auto tv = [&](int r, int q) -> double {
         if (REV) {
             r = n - 1 - r;
             q = n - 1 - q;
         }
-        return TT ? t[q + r * ldt] : t[r + q * ldt];
+        const unsigned off = TT ? (unsigned)q + __umul24((unsigned)r, ldt32)
+                                : (unsigned)r + __umul24((unsigned)q, ldt32);
+        return t[off];
     };
     double *D = smem + lb * 16 * 17;
     const bool live = j0 < n;
@@ -703,12 +711,17 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
     const size_t ldt = u.lda, ldz = u.ldc;
     MU_G double *z = gp(u.c);
     const double alpha = u.alpha;
+    // 32-bit offsets with 24-bit multiplies (full rate, unlike 64-bit
+    // products): the lowering guarantees (n + 1) * ldt < 2^31 for triangles
+    const unsigned ldt32 = (unsigned)ldt;
     auto tv = [&](int r, int c) -> double {
         if (REV) {
             r = n - 1 - r;
             c = n - 1 - c;
         }
-        return TT ? t[c + r * ldt] : t[r + c * ldt];
+        const unsigned off = TT ? (unsigned)c + __umul24((unsigned)r, ldt32)
+                                : (unsigned)r + __umul24((unsigned)c, ldt32);
+        return t[off];
     };
     auto zp = [&](int r, int c) -> MU_G double * {
         if (REV)

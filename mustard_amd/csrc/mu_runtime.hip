@@ -709,4 +709,160 @@ int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Column maxima of |M| (m x n): out[j] = max_i |M(i, j)|.  Non-negative
+// doubles order like their bit patterns, so an integer atomic max combines
+// the per-thread maxima; out must be zeroed first.
+__global__ void k_colmax_abs(const double *M, size_t ld, int m, int n, unsigned long long *out)
+{
+    const int j = blockIdx.y;
+    double v = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+        v = fmax(v, fabs(M[i + j * ld]));
+    __shared__ double red[256];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMax(out + j, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+int colmax_abs(const double *M, int ld, int m, int n, double *out, hipStream_t s)
+{
+    if (n <= 0)
+        return 0;
+    unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, s, o, n, 0ull);
+    const int gx = std::max(1, std::min(256, (m + 255) / 256));
+    hipLaunchKernelGGL(k_colmax_abs, dim3(gx, n), dim3(256), 0, s, M, (size_t)ld, m, n, o);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ||A||_inf = max_i sum_j |A(i, j)|: one thread per row (a wave reads 64
+// consecutive rows of a column: coalesced), then an atomic max.
+__global__ void k_rowsum_max(const double *A, size_t lda, int m, int n, unsigned long long *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sum = 0.0;
+    if (i < m)
+        for (int j = 0; j < n; j++)
+            sum += fabs(A[i + j * lda]);
+    __shared__ double red[256];
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+int inf_norm(const double *A, int lda, int m, int n, double *out, hipStream_t s)
+{
+    unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, s, o, 1, 0ull);
+    hipLaunchKernelGGL(k_rowsum_max, dim3((m + 255) / 256), dim3(256), 0, s, A, (size_t)lda, m,
+                       n, o);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// X += D (m x n)
+__global__ void k_add_matrix(double *X, size_t ldx, const double *D, size_t ldd, int m, int n)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)m * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % m, j = k / m;
+        X[i + j * ldx] += D[i + j * ldd];
+    }
+}
+
+int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_add_matrix, dim3(1024), dim3(256), 0, s, X, (size_t)ldx, D, (size_t)ldd,
+                       m, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Conditioned test matrices: A = diag(s), then A <- H A H for each reflector.
+__global__ void k_diag_geometric(double *A, size_t lda, int n, double kappa)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % n, j = k / n;
+        A[i + j * lda] = i == j ? (n > 1 ? pow(kappa, -(double)i / (double)(n - 1)) : 1.0) : 0.0;
+    }
+}
+
+// acc[0] += x . y
+__global__ void k_dot(const double *x, const double *y, int n, double *acc)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += x[i] * y[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicAdd(acc, red[0]);
+}
+
+// A <- H A H with H = I - tau v v^T, tau = 2 / v.v, given w = A v and
+// c = v.w (sc[0] = v.v, sc[1] = v.w):
+//   A - tau (v w^T + w v^T) + tau^2 c v v^T.
+// The products are rounded separately (contraction into fused multiply-adds
+// is off here), so A(i, j) and A(j, i) stay bit-for-bit equal.
+__global__ void k_reflect2(double *A, size_t lda, int n, const double *v, const double *w,
+                           const double *sc)
+{
+#pragma clang fp contract(off)
+    const double tau = 2.0 / sc[0], c = sc[1];
+    const double t2c = (tau * tau) * c;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % n, j = k / n;
+        const double p1 = v[i] * w[j], p2 = w[i] * v[j];
+        const double s1 = p1 + p2;
+        const double s2 = v[i] * v[j];
+        const double d1 = tau * s1, d2 = t2c * s2;
+        A[i + j * lda] = (A[i + j * lda] - d1) + d2;
+    }
+}
+
+int gen_cond_start(double *A, int n, int lda, double kappa, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_diag_geometric, dim3(4096), dim3(256), 0, s, A, (size_t)lda, n, kappa);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int dot_acc(const double *x, const double *y, int n, double *acc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot, dim3(std::max(1, std::min(1024, (n + 255) / 256))), dim3(256), 0, s,
+                       x, y, n, acc);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int reflect2(double *A, int lda, int n, const double *v, const double *w, const double *sc,
+             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reflect2, dim3(4096), dim3(256), 0, s, A, (size_t)lda, n, v, w, sc);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace mu

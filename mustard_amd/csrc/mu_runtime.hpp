@@ -72,5 +72,16 @@ int sub_frob2(const double *A, int lda, const double *B, int ldb, int m, int n, 
               hipStream_t s);
 int sum_abs(const double *A, int lda, int m, int n, double *acc, hipStream_t s);
 int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, hipStream_t s);
+// out[j] = max_i |M(i, j)| for the n columns of M (m x n)
+int colmax_abs(const double *M, int ld, int m, int n, double *out, hipStream_t s);
+// *out = ||A||_inf (max row sum of |A|)
+int inf_norm(const double *A, int lda, int m, int n, double *out, hipStream_t s);
+// X += D
+int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipStream_t s);
+// conditioned test matrices: A = diag(kappa^(-i/(n-1))); acc += x.y; A <- H A H
+int gen_cond_start(double *A, int n, int lda, double kappa, hipStream_t s);
+int dot_acc(const double *x, const double *y, int n, double *acc, hipStream_t s);
+int reflect2(double *A, int lda, int n, const double *v, const double *w, const double *sc,
+             hipStream_t s);
 
 }  // namespace mu

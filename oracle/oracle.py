@@ -231,6 +231,44 @@ def chol_solve(F, B):
     return X
 
 
+def gen_cond(n: int, kappa: float, seed: int, nrefl: int = 4, _tau_scale: float = 2.0):
+    """Restatement of mu_gen_cond: A = H_r .. H_1 diag(s) H_1 .. H_r with
+    s_i = kappa^(-i/(n-1)) and H = I - 2 v v^T / (v.v), v uniform [-1, 1)
+    from stream seed + r; each step A <- H A H is applied as
+    A - tau (v w^T + w v^T) + tau^2 (v.w) v v^T with w = A v.  The eigenvalues
+    of the result are exactly s (the closed form the tests check).
+    `_tau_scale` exists only so a test can plant a wrong reflector."""
+    s = kappa ** (-np.arange(n) / (n - 1)) if n > 1 else np.ones(1)
+    A = np.diag(s)
+    for r in range(nrefl):
+        v = gen_uniform((n,), seed + r, -1.0, 1.0)
+        w = A @ v
+        tau = _tau_scale / (v @ v)
+        c = v @ w
+        A = A - tau * (np.outer(v, w) + np.outer(w, v)) + tau * tau * c * np.outer(v, v)
+    return np.asfortranarray(A)
+
+
+def backward_error(A, X, B):
+    """Normwise backward error, max over columns j of
+    ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)."""
+    R = B - A @ X
+    an = np.max(np.sum(np.abs(A), axis=1))
+    num = np.max(np.abs(R), axis=0)
+    den = an * np.max(np.abs(X), axis=0) + np.max(np.abs(B), axis=0)
+    return float(np.max(num / den))
+
+
+def solve_fp64(kind: str, A, B):
+    """FP64 LAPACK-path solution: no-pivot getrf + getrs, or potrf + potrs."""
+    F = np.array(A, dtype=np.float64, order="F", copy=True)
+    if kind == "lu":
+        assert getrf_nopiv(F) == 0
+        return lu_solve(F, B)
+    assert potrf_lower(F) == 0
+    return chol_solve(F, B)
+
+
 # --------------------------------------------------------------------------
 # task DAG restatement
 # --------------------------------------------------------------------------

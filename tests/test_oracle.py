@@ -285,3 +285,35 @@ def test_task_dag_fixture_and_counts(oracle):
         assert len(ops) == (4900 if kind == "lu" else 2600)
         assert sum(map(len, deps)) == edges
         assert oracle.critical_path(deps) == 70
+
+
+@pytest.mark.parametrize("kappa", [1.0, 1e3, 1e6])
+def test_gen_cond_spectrum(oracle, kappa):
+    """The conditioned generator's closed form: eigenvalues are exactly
+    kappa^(-i/(n-1)), the matrix is symmetric, and a wrong reflector scale
+    (planted) breaks the spectrum."""
+    n = 120
+    A = oracle.gen_cond(n, kappa, 11, nrefl=4)
+    s = np.sort(kappa ** (-np.arange(n) / (n - 1)))
+    assert np.allclose(A, A.T, rtol=0, atol=1e-15)
+    ev = np.sort(np.linalg.eigvalsh(A))
+    assert np.max(np.abs(ev - s)) <= 1e-13
+    assert abs(np.linalg.cond(A) / kappa - 1) < 1e-6
+    # not diagonal: the reflectors really mix it (kappa = 1 gives H I H = I)
+    if kappa > 1:
+        assert np.max(np.abs(A - np.diag(np.diag(A)))) > 1e-3 / kappa
+    bad = oracle.gen_cond(n, kappa, 11, nrefl=4, _tau_scale=1.0)
+    assert np.max(np.abs(np.sort(np.linalg.eigvalsh(bad)) - s)) > 1e-6 or kappa == 1.0
+
+
+def test_backward_error_and_fp64_solve(oracle):
+    n = 80
+    A = oracle.gen_cond(n, 1e3, 3)
+    B = oracle.gen_uniform((n, 4), 5, -1.0, 1.0)
+    for kind in ("lu", "chol"):
+        X = oracle.solve_fp64(kind, A, B)
+        assert np.allclose(X, np.linalg.solve(A, B), rtol=1e-9, atol=0)
+        be = oracle.backward_error(A, X, B)
+        assert be <= 10 * n * EPS
+        # a perturbed solution has a much larger backward error
+        assert oracle.backward_error(A, X * (1 + 1e-6), B) > 100 * be
