@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--mode", default="tiled", choices=["tiled", "stream"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-residual", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision + IR solves")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven factorizations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs the tiled factorizations on a "
@@ -227,6 +229,98 @@ def main():
             r = mu.scaled_residual(A0, A, n, chol=(kind == "cholesky"))
             out[kind]["scaled_residual"] = round(r, 4)
             log(f"[bench] {kind} scaled residual {r:.4f}")
+    # Mixed precision + FP64 refinement (the metric's second half): A X = B
+    # with 16 right-hand sides on an SPD matrix of condition number 1e3
+    # (A = Q diag(s) Q^T, Householder Q), time-to-solution of the plain FP64
+    # tiled solve vs FP32 (and BF16) trailing updates + refinement, each to
+    # the same stopping test and reported with its backward error.
+    if not args.no_mixed:
+        nrhs, kappa = 16, 1e3
+        Ac = mu.gen_cond(n, kappa, 420)
+        Bm = torch.empty((nrhs, n), dtype=torch.float64, device="cuda")
+        mu.call("mu_gen_uniform", mu.ptr(Bm), n * nrhs, 421, -1.0, 1.0, None)
+        torch.cuda.synchronize()
+        mixed = {"rhs": nrhs, "kappa": kappa, "bound_10_n_eps": 10 * n * 2.0 ** -52}
+        for kind in ("lu", "chol"):
+            r = {}
+            for prec in ("fp64", "fp32", "bf16"):
+                best = None
+                for _ in range(2):  # the first run warms the code paths
+                    _, res = mu.solve_ir(kind, Ac, Bm, n, nrhs, t, prec=prec, fallback=True)
+                    if best is None or res["total_ms"] < best["total_ms"]:
+                        best = res
+                r[prec] = {"ms": round(best["total_ms"], 3), "fact_ms": round(best["fact_ms"], 3),
+                           "iters": best["iters"], "converged": best["converged"],
+                           "fell_back": best["fell_back"], "berr": best["berr"],
+                           "berr_unrefined": best["berr0"]}
+                log(f"[bench] {kind} {prec}+IR: {best['total_ms']:.1f} ms, {best['iters']} steps, "
+                    f"berr {best['berr']:.2e}")
+            r["speedup_fp32_ir"] = round(r["fp64"]["ms"] / r["fp32"]["ms"], 3)
+            r["speedup_bf16_ir"] = round(r["fp64"]["ms"] / r["bf16"]["ms"], 3)
+            mixed[kind] = r
+        out["mixed_ir"] = mixed
+        del Ac, Bm
+    # Batched uneven factorizations in one graph launch: 4096 SPD Cholesky
+    # problems with n uniform in [32, 512] and 512 diagonally dominant LU
+    # problems with n uniform in [64, 1024] (seed 420), tile counts from the
+    # autotuner's table; aggregate GFLOP/s, and per-problem scaled residuals
+    # and oracle comparison on a sample.
+    if not args.no_batch:
+        import numpy as np
+        from mustard_amd import autotune
+        from mustard_amd.batch import Batch, make_problems, unpack
+        rng = np.random.default_rng(420)
+        kinds = ["chol"] * 4096 + ["lu"] * 512
+        ns = [int(x) for x in rng.integers(32, 513, 4096)] + \
+             [int(x) for x in rng.integers(64, 1025, 512)]
+        try:
+            table = autotune.load()
+            tsrc = os.path.relpath(autotune.DEFAULT_TABLE, ROOT)
+        except OSError:
+            table, tsrc = {}, "none (one tile per problem; the lowering nests)"
+        tiles = [autotune.choose(k, nn, table) for k, nn in zip(kinds, ns)]
+        host = make_problems(kinds, ns, 420)
+        t0 = time.perf_counter()
+        bt = Batch(kinds, ns, tiles)
+        build_s = time.perf_counter() - t0
+        ms = []
+        for r in range(3):
+            bt.load(host)
+            torch.cuda.synchronize()
+            with torch.cuda.stream(stream):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                bt.launch(stream)
+                e1.record(stream)
+            stream.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        assert not bt.infos().any()
+        worst = 0.0
+        sample = list(range(0, len(ns), 36))
+        for p in sample:
+            A = unpack(host, ns, p)
+            F = bt.factor(p)
+            nn = ns[p]
+            if kinds[p] == "lu":
+                Lf = np.tril(F, -1) + np.eye(nn)
+                R = A - Lf @ np.triu(F)
+            else:
+                Lf = np.tril(F)
+                R = A - Lf @ Lf.T
+            worst = max(worst, np.linalg.norm(R) / (np.linalg.norm(A) * nn * 2.0 ** -52))
+        best = min(ms)
+        out["batched"] = {"problems": len(ns), "cholesky": 4096, "lu": 512,
+                          "gflop": round(bt.flops * 1e-9, 3), "ms": round(best, 3),
+                          "gflops": round(bt.flops / (best * 1e-3) * 1e-9, 1),
+                          "levels": bt.exec.stats()["levels"], "graph_launches": 1,
+                          "tile_table": tsrc, "record_and_build_s": round(build_s, 2),
+                          "max_scaled_residual_sample": round(worst, 4),
+                          "sample": f"{len(sample)} problems (every 36th)"}
+        log(f"[bench] batched: {len(ns)} problems, {bt.flops * 1e-9:.1f} GFLOP in {best:.2f} ms "
+            f"({bt.flops / (best * 1e-3) * 1e-9:.0f} GFLOP/s), {out['batched']['levels']} levels, "
+            f"max scaled residual {worst:.3f}")
+        bt.close()
+        del host
     if not args.no_cpu:
         log("[bench] CPU baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -199,6 +199,17 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
 int mu_record_tiled_solve(mu_dag_t dag, const double *F, int n, int t, int ldf, int kind,
                           double *X, int ldx, int nrhs);
 
+/* Batched uneven factorizations: records nprob independent tiled LU
+ * (kinds[p] = MU_SOLVE_LU) or Cholesky (MU_SOLVE_CHOL) problems of order
+ * ns[p] at As[p] (leading dimension ldas[p], tiles[p] tiles per dimension,
+ * ns[p] % tiles[p] == 0) into one DAG, each under its own tag; one executable
+ * then runs them all in one graph launch, every level kernel mixing the
+ * items of all problems.  info_dev (nprob ints, or NULL) receives each
+ * problem's info; prec applies to every problem's trailing updates.  No
+ * counterpart in the reference (one factorization per process there). */
+int mu_record_batch(mu_dag_t dag, int nprob, const int *kinds, const int *ns, double *const *As,
+                    const int *ldas, const int *tiles, int *info_dev, int prec);
+
 /* ---- Mixed precision + FP64 iterative refinement ------------------------
  * No counterpart in the reference: the precision form of the tiled
  * factorizations.  Solves A X = B (A n x n, B and X n x nrhs, device
@@ -232,6 +243,18 @@ typedef struct {
 } mu_ir_result_t;
 int mu_solve_ir(const double *A, int n, int lda, const double *B, int ldb, double *X, int ldx,
                 int nrhs, const mu_ir_opts_t *opts, mu_ir_result_t *res, void *stream);
+
+/* ---- Reverse mode of the tiled Cholesky ------------------------------------
+ * No counterpart in the reference.  Given the factor L (lower triangle used)
+ * of A = L L^T and the gradient Lbar of a scalar loss with respect to the
+ * lower triangle of L, writes the symmetric gradient with respect to A,
+ * G = (Abar + Abar^T)/2 with Abar = L^-T Phi(L^T Lbar) L^-1 (Phi: lower
+ * triangle, diagonal halved; Murray, arXiv:1602.07527), computed by three
+ * tiled DAGs of GEMM and TRSM tasks over t x t tiles (n % t == 0).  `prec`
+ * sets the precision of the GEMM tasks (MU_PREC_*).  L and Lbar are not
+ * modified; G may not alias them.  Workspace: 3 n^2 doubles, library-owned. */
+int mu_chol_backward(const double *L, int ldl, const double *Lbar, int ldlb, double *G, int ldg,
+                     int n, int t, int prec, void *stream);
 
 /* Test matrices of a given condition number (the mixed-precision inputs):
  * A = Q diag(s) Q^T with s_i = kappa^(-i/(n-1)) (so ||A||_2 = 1 and

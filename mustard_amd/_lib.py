@@ -112,6 +112,8 @@ SIGNATURES = {
     "mu_gen_uniform": (_i, [_vp, _sz, _u64, _d, _d, _vp]),
     "mu_gen_spd": (_i, [_vp, _i, _i, _u64, _vp, _vp]),
     "mu_gen_cond": (_i, [_vp, _i, _i, _d, _i, _u64, _vp, _vp]),
+    "mu_chol_backward": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _i, _i, _vp]),
+    "mu_record_batch": (_i, [_vp, _i, _ip, _ip, ctypes.POINTER(_vp), _ip, _ip, _vp, _i]),
     "mu_record_tiled_solve": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _i, _i]),
     "mu_solve_ir": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _i, ctypes.POINTER(IROpts),
                          ctypes.POINTER(IRResult), _vp]),

@@ -875,6 +875,29 @@ int mu_record_tiled_solve(mu_dag_t dag, const double *F, int n, int t, int ldf, 
     return rc ? rc : rc2;
 }
 
+int mu_record_batch(mu_dag_t dag, int nprob, const int *kinds, const int *ns, double *const *As,
+                    const int *ldas, const int *tiles, int *info_dev, int prec)
+{
+    if (!dag || nprob < 0 || (nprob > 0 && (!kinds || !ns || !As || !ldas || !tiles)))
+        return fail(MU_EINVAL, "mu_record_batch: bad arguments");
+    for (int p = 0; p < nprob; p++) {
+        if (kinds[p] != MU_SOLVE_LU && kinds[p] != MU_SOLVE_CHOL)
+            return fail(MU_EINVAL, "mu_record_batch: problem " + std::to_string(p) +
+                                       ": kind must be MU_SOLVE_LU or MU_SOLVE_CHOL");
+        int rc = mu_dag_set_tag(dag, p + 1);  // tiles of different problems never alias
+        if (rc)
+            return rc;
+        rc = kinds[p] == MU_SOLVE_LU
+                 ? mu_record_tiled_lu(dag, As[p], ns[p], tiles[p], ldas[p],
+                                      info_dev ? info_dev + p : nullptr, prec)
+                 : mu_record_tiled_cholesky(dag, As[p], ns[p], tiles[p], ldas[p],
+                                            info_dev ? info_dev + p : nullptr, prec);
+        if (rc)
+            return fail(rc, "mu_record_batch: problem " + std::to_string(p) + ": " + g_err);
+    }
+    return mu_dag_set_tag(dag, 0);
+}
+
 // ---- generators -------------------------------------------------------------
 void mu_gen_ref_host(double *h_A, size_t n)
 {

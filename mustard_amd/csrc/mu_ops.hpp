@@ -775,21 +775,41 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
             }
         }
         load_inv(wn, jb + 16);
+        // Row blocks in groups of four under one (wave-uniform) test, so the
+        // four MFMA chains of a group interleave: the accumulators start as
+        // the Y rows and take -T[rb, jb] X_jb.  A block of a live group that
+        // lies past the panel reads clamped rows and is not written back.
 #pragma unroll
-        for (int i = 0; i < MAXB; i++) {
-            const int rb = jb + 16 + 16 * wave + 64 * i;
-            if (rb < np) {
-                v4d acc = {0.0, 0.0, 0.0, 0.0};
+        for (int g = 0; g < MAXB / 4; g++) {
+            const int rb0 = jb + 16 + 16 * wave + 256 * g;
+            if (rb0 < np) {
+                v4d acc[4];
 #pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    const bool in = rb + li < n && jb + 4 * kk + lk < n;
-                    acc = mfma_f64(in ? fa[i][kk] : 0.0, xs[kk], acc);
+                for (int q = 0; q < 4; q++) {
+                    const int rr = min(rb0 + 64 * q, np - 16);
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        acc[q][e] = Y[(rr + 4 * e + lk) * 16 + li];
                 }
 #pragma unroll
-                for (int e = 0; e < 4; e++)
-                    Y[(rb + 4 * e + lk) * 16 + li] -= acc[e];
+                for (int kk = 0; kk < 4; kk++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const bool in = rb0 + 64 * q + li < n && jb + 4 * kk + lk < n;
+                        acc[q] = mfma_f64(in ? -fa[4 * g + q][kk] : 0.0, xs[kk], acc[q]);
+                    }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int rb = rb0 + 64 * q;
+                    if (rb < np)
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+                            Y[(rb + 4 * e + lk) * 16 + li] = acc[q][e];
+                }
             }
-            load_block(fn[i], jb + 16, i);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                load_block(fn[4 * g + q], jb + 16, 4 * g + q);
         }
         lds_barrier();
     };

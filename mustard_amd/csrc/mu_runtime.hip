@@ -845,6 +845,43 @@ __global__ void k_reflect2(double *A, size_t lda, int n, const double *v, const 
     }
 }
 
+// dst = tril(src) with the diagonal scaled by dscale and zeros above it
+__global__ void k_tril_scaled(const double *src, size_t lds, double *dst, size_t ldd, int n,
+                              double dscale)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % n, j = k / n;
+        const double v = src[i + j * lds];
+        dst[i + j * ldd] = i > j ? v : i == j ? dscale * v : 0.0;
+    }
+}
+
+int tril_scaled(const double *src, int lds, double *dst, int ldd, int n, double dscale,
+                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tril_scaled, dim3(4096), dim3(256), 0, s, src, (size_t)lds, dst,
+                       (size_t)ldd, n, dscale);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// dst = (src + src^T) / 2 (out of place)
+__global__ void k_sym_avg(const double *src, size_t lds, double *dst, size_t ldd, int n)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)n * n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % n, j = k / n;
+        dst[i + j * ldd] = 0.5 * (src[i + j * lds] + src[j + i * lds]);
+    }
+}
+
+int sym_avg(const double *src, int lds, double *dst, int ldd, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sym_avg, dim3(4096), dim3(256), 0, s, src, (size_t)lds, dst, (size_t)ldd,
+                       n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int gen_cond_start(double *A, int n, int lda, double kappa, hipStream_t s)
 {
     hipLaunchKernelGGL(k_diag_geometric, dim3(4096), dim3(256), 0, s, A, (size_t)lda, n, kappa);

@@ -269,6 +269,36 @@ def solve_fp64(kind: str, A, B):
     return chol_solve(F, B)
 
 
+def chol_backward(L, Lbar):
+    """Restated reverse mode of the Cholesky factorization (Murray,
+    arXiv:1602.07527, eq. 10): G = (Abar + Abar^T)/2 with
+    Abar = L^-T Phi(L^T Lbar) L^-1, Phi = lower triangle with the diagonal
+    halved.  Dense FP64."""
+    import scipy.linalg as sl
+    L = np.tril(L)
+    Lbar = np.tril(Lbar)
+    X = L.T @ Lbar
+    P = np.tril(X) - 0.5 * np.diag(np.diag(X))
+    W = sl.solve_triangular(L.T, P.T, lower=False).T       # W L = P
+    Abar = sl.solve_triangular(L.T, W, lower=False)        # L^T Abar = W
+    return 0.5 * (Abar + Abar.T)
+
+
+def c3_loss_grad(A, B):
+    """Loss sum(log diag L) + 0.5 ||L^-1 B||_F^2 with A = L L^T, which equals
+    0.5 log det A + 0.5 tr(B^T A^-1 B), its Lbar, and its closed-form symmetric
+    gradient 0.5 A^-1 - 0.5 A^-1 B B^T A^-1."""
+    import scipy.linalg as sl
+    L = np.linalg.cholesky(A)
+    Y = sl.solve_triangular(L, B, lower=True)
+    loss = float(np.sum(np.log(np.diag(L))) + 0.5 * np.sum(Y * Y))
+    Z = sl.solve_triangular(L.T, Y, lower=False)
+    Lbar = np.diag(1.0 / np.diag(L)) - np.tril(Z @ Y.T)
+    Ai = np.linalg.inv(A)
+    G = 0.5 * Ai - 0.5 * Ai @ B @ B.T @ Ai
+    return loss, L, Lbar, G
+
+
 # --------------------------------------------------------------------------
 # task DAG restatement
 # --------------------------------------------------------------------------

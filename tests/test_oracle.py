@@ -317,3 +317,29 @@ def test_backward_error_and_fp64_solve(oracle):
         assert be <= 10 * n * EPS
         # a perturbed solution has a much larger backward error
         assert oracle.backward_error(A, X * (1 + 1e-6), B) > 100 * be
+
+
+def test_chol_backward_oracle_pinned(oracle):
+    """The restated Cholesky adjoint equals the closed-form gradient of the
+    config loss, agrees with central differences along random symmetric
+    directions, and a planted mistake (Phi without halving the diagonal)
+    is caught."""
+    rng = np.random.default_rng(3)
+    n, m = 64, 32
+    A = oracle.gen_spd(n, 5)
+    B = rng.uniform(-1, 1, (n, m))
+    loss, L, Lbar, Gc = oracle.c3_loss_grad(A, B)
+    G = oracle.chol_backward(L, Lbar)
+    assert np.max(np.abs(G - Gc)) <= 1e-12 * np.max(np.abs(Gc))
+    h = 1e-6
+    for _ in range(20):
+        E = rng.uniform(-1, 1, (n, n))
+        E = E + E.T
+        fd = (oracle.c3_loss_grad(A + h * E, B)[0] - oracle.c3_loss_grad(A - h * E, B)[0]) / (2 * h)
+        assert abs(fd - np.sum(G * E)) <= 1e-6 * max(1.0, abs(fd))
+    X = L.T @ np.tril(Lbar)
+    bad_P = np.tril(X)  # diagonal not halved
+    import scipy.linalg as sl
+    W = sl.solve_triangular(L.T, bad_P.T, lower=False).T
+    Ab = sl.solve_triangular(L.T, W, lower=False)
+    assert np.max(np.abs(0.5 * (Ab + Ab.T) - Gc)) > 1e-6
