@@ -216,11 +216,10 @@ def main():
                            "kernel": "k_batch_gemm", "launches": g["launches"],
                            "avg_launch_ms": round(g["ms"] / g["launches"], 4),
                            "avg_launch_gflop": round(g["flops"] / g["launches"] * 1e-9, 3)}
-        o = per.get("k_batch")
-        if o:
-            out["roofline"]["other_kernel"] = {
-                "kernel": "k_batch", "launches": o["launches"], "ms": round(o["ms"], 3),
-                "tflops": round(o["flops"] / (o["ms"] * 1e-3) * 1e-12, 3)}
+        out["roofline"]["other_kernels"] = {
+            name: {"launches": o["launches"], "ms": round(o["ms"], 3),
+                   "tflops": round(o["flops"] / (o["ms"] * 1e-3) * 1e-12, 3)}
+            for name, o in per.items() if name != "k_batch_gemm"}
         for name, p in per.items():
             log(f"[bench] {name}: {p['launches']} launches, {p['ms']:.1f} ms, "
                 f"{p['flops'] / (p['ms'] * 1e-3) * 1e-12:.2f} TFLOP/s")

@@ -163,11 +163,13 @@ int mu_exec_level_items(mu_exec_t exec, int level, long long *items);
 /* Items of one level per operation: counts[MU_OP_*] (7 entries). */
 int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 /* The kernel one level runs on (MU_KERNEL_GENERIC: k_batch, any item kind;
- * MU_KERNEL_GEMM: k_batch_gemm, GEMM/SYRK blocks only) and the algorithmic
+ * MU_KERNEL_GEMM: k_batch_gemm, GEMM/SYRK blocks only; MU_KERNEL_SMALL:
+ * k_batch_small, diagonal GETRF/POTRF of order <= 128 only) and the algorithmic
  * flops of its units (GEMM 2mnk, SYRK n(n+1)k, TRSM m^2 n, GETRF 2m^3/3,
  * POTRF m^3/3).  Either output may be NULL. */
 #define MU_KERNEL_GENERIC 0
 #define MU_KERNEL_GEMM 1
+#define MU_KERNEL_SMALL 2
 int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);

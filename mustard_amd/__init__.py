@@ -203,7 +203,7 @@ class Exec:
         """(kernel name, algorithmic flops) of one level of a tiled executable."""
         k, f = ctypes.c_int(), ctypes.c_double()
         call("mu_exec_level_info", self.h, level, ctypes.byref(k), ctypes.byref(f))
-        return ("k_batch_gemm" if k.value == 1 else "k_batch"), f.value
+        return {1: "k_batch_gemm", 2: "k_batch_small"}.get(k.value, "k_batch"), f.value
 
     def level_items(self, level):
         n = ctypes.c_longlong()

@@ -588,7 +588,9 @@ int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops)
     if (!e || level < 0 || level >= e->levels)
         return fail(MU_EINVAL, "mu_exec_level_info: tiled executables only, level out of range");
     if (kernel)
-        *kernel = e->level_gemm[level] ? MU_KERNEL_GEMM : MU_KERNEL_GENERIC;
+        *kernel = e->level_gemm[level] == mu::LEVEL_GEMM    ? MU_KERNEL_GEMM
+                  : e->level_gemm[level] == mu::LEVEL_SMALL ? MU_KERNEL_SMALL
+                                                            : MU_KERNEL_GENERIC;
     if (flops)
         *flops = e->level_flops[level];
     return MU_OK;

@@ -875,6 +875,23 @@ __device__ __forceinline__ void small_store(const SmallTile &T, MU_G double *a, 
         }
 }
 
+// The 16 owners of column c (tj == c % 16) are 16 consecutive lanes of one
+// wave, and the owner of the diagonal entry (ti == c % 16) is one of them, so
+// they take the pivot from it by a lane shuffle: no LDS hand-off and no
+// barrier before the column is scaled.  The pivot row (row owners) and the
+// scaled column (column owners) are then published together, and one barrier
+// per column separates publication from the rank-1 update.
+__device__ __forceinline__ double col_pivot(const SmallTile &T, int blk, int own)
+{
+    double v = 0.0;
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+        if (x == blk)
+            v = T.r[x][x];
+    const int lane = threadIdx.x & 63;
+    return __shfl(v, (lane & ~15) | own, 64);
+}
+
 __device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc &u, double *smem)
 {
     const int n = u.m, tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
@@ -882,7 +899,6 @@ __device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc
     const size_t lda = u.ldc;
     double *rowbuf = smem;        // [2][128]
     double *colbuf = smem + 256;  // [2][128]
-    double *pivbuf = smem + 512;  // [2]
     SmallTile T;
     small_load(T, a, lda, n, ti, tj, false);
     for (int c = 0; c < n; c++) {
@@ -896,18 +912,15 @@ __device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc
                     for (int y = 0; y < 8; y++) {
                         const int col = tj + 16 * y;
                         rowbuf[p * 128 + col] = col > c ? T.r[x][y] : 0.0;
-                        if (col == c) {
-                            pivbuf[p] = T.r[x][y];
-                            if (T.r[x][y] == 0.0)
-                                report_info(u, c);
-                        }
                     }
         }
-        lds_barrier();
         if (tj == own) {
             // owners of column c scale it below the pivot (by the reciprocal,
             // as LAPACK's dgetf2 does) and publish it
-            const double rp = 1.0 / pivbuf[p];
+            const double piv = col_pivot(T, blk, own);
+            if (ti == own && piv == 0.0)
+                report_info(u, c);
+            const double rp = 1.0 / piv;
 #pragma unroll
             for (int y = 0; y < 8; y++)
                 if (y == blk)
@@ -942,24 +955,15 @@ __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc
     MU_G double *a = gp(u.c);
     const size_t lda = u.ldc;
     double *colbuf = smem + 256;  // [2][128]
-    double *pivbuf = smem + 512;  // [2]
     SmallTile T;
     small_load(T, a, lda, n, ti, tj, true);
     for (int c = 0; c < n; c++) {
         const int p = c & 1, own = c & 15, blk = c >> 4;
-        if (ti == own && tj == own) {
-#pragma unroll
-            for (int x = 0; x < 8; x++)
-                if (x == blk) {
-                    const double v = T.r[x][x];
-                    if (!(v > 0.0))
-                        report_info(u, c);
-                    pivbuf[p] = sqrt(v);
-                }
-        }
-        lds_barrier();
         if (tj == own) {
-            const double d = pivbuf[p], rd = 1.0 / d;
+            const double v = col_pivot(T, blk, own);
+            if (ti == own && !(v > 0.0))
+                report_info(u, c);
+            const double d = sqrt(v), rd = 1.0 / d;
 #pragma unroll
             for (int y = 0; y < 8; y++)
                 if (y == blk)
@@ -1051,6 +1055,16 @@ __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDe
     case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
     default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
     }
+}
+
+// Small GETRF/POTRF item (the only kinds k_batch_small runs).
+__device__ __attribute__((always_inline)) inline void run_small_item(const UnitDesc &u,
+                                                                     double *smem)
+{
+    if (u.op == OP_GETRF)
+        getrf_small(u, smem);
+    else
+        potrf_small(u, smem);
 }
 
 // Execute local item `it` of unit `u`.  Always inlined, so that LDS stays

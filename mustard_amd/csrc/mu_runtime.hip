@@ -72,7 +72,18 @@ __global__ MU_GEMM_KERNEL void k_batch_gemm(const UnitDesc *units, const uint32_
     run_gemm_item(d, (int)(item - d.item_base), smem);
 }
 
+__global__ __launch_bounds__(WG_THREADS) void k_batch_small(const UnitDesc *units,
+                                                            const uint32_t *item_unit,
+                                                            const uint32_t *items)
+{
+    __shared__ double smem[LDS_SMALL];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    run_small_item(d, smem);
+}
+
 static bool gemm_only_op(int op) { return op == OP_GEMM || op == OP_SYRK; }
+static bool small_only_op(int op) { return op == OP_GETRF || op == OP_POTRF; }
 
 // Algorithmic flops of one lowered unit (the whole operation its items cover).
 static double unit_flops(const UnitDesc &u)
@@ -328,14 +339,19 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         for (uint32_t u = 0; u < nu; u++)
             if (L.units[u].nitems > 0)
                 e->level_flops[level[u]] += unit_flops(L.units[u]);
-        // a level whose items are all GEMM/SYRK blocks runs on k_batch_gemm
-        e->level_gemm.assign(nlev, 1);
-        for (int l = 0; l < nlev; l++)
-            for (uint32_t it : lv[l])
-                if (!gemm_only_op(L.units[item_unit[it]].op)) {
-                    e->level_gemm[l] = 0;
-                    break;
-                }
+        // a level whose items are all GEMM/SYRK blocks runs on k_batch_gemm,
+        // one of only small diagonal factorizations on k_batch_small (both
+        // with small LDS, so several workgroups share a CU)
+        e->level_gemm.assign(nlev, LEVEL_GENERIC);
+        for (int l = 0; l < nlev; l++) {
+            bool g = true, sm = true;
+            for (uint32_t it : lv[l]) {
+                const int op = L.units[item_unit[it]].op;
+                g = g && gemm_only_op(op);
+                sm = sm && small_only_op(op);
+            }
+            e->level_gemm[l] = g ? LEVEL_GEMM : sm ? LEVEL_SMALL : LEVEL_GENERIC;
+        }
         HIPCHK(hipGraphCreate(&e->graph, 0));
         hipGraphNode_t prev = nullptr;
         for (int l = 0; l < nlev; l++) {
@@ -343,8 +359,9 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
             void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
             hipKernelNodeParams kp;
             std::memset(&kp, 0, sizeof kp);
-            kp.func = e->level_gemm[l] ? reinterpret_cast<void *>(k_batch_gemm)
-                                       : reinterpret_cast<void *>(k_batch);
+            kp.func = e->level_gemm[l] == LEVEL_GEMM    ? reinterpret_cast<void *>(k_batch_gemm)
+                      : e->level_gemm[l] == LEVEL_SMALL ? reinterpret_cast<void *>(k_batch_small)
+                                                        : reinterpret_cast<void *>(k_batch);
             kp.gridDim = dim3((unsigned)lv[l].size());
             kp.blockDim = dim3(WG_THREADS);
             kp.kernelParams = args;
@@ -488,8 +505,10 @@ int exec_launch_level(Exec *e, int level, hipStream_t s, std::string &err)
     const uint32_t *diu = (const uint32_t *)e->level_args[1];
     const uint32_t *first = (const uint32_t *)e->level_args[2] + e->level_off[level];
     const unsigned n = (unsigned)(e->level_off[level + 1] - e->level_off[level]);
-    if (e->level_gemm[level])
+    if (e->level_gemm[level] == LEVEL_GEMM)
         hipLaunchKernelGGL(k_batch_gemm, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
+    else if (e->level_gemm[level] == LEVEL_SMALL)
+        hipLaunchKernelGGL(k_batch_small, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
     else
         hipLaunchKernelGGL(k_batch, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
     HIPCHK(hipGetLastError());

@@ -21,6 +21,8 @@ struct ExecOpts {
     int trace = 0;             // persistent: record per-item wall-clock stamps
 };
 
+enum LevelKernel : char { LEVEL_GENERIC = 0, LEVEL_GEMM = 1, LEVEL_SMALL = 2 };
+
 struct Exec {
     int mode = MODE_PERSISTENT;
     int device = 0;
@@ -42,7 +44,7 @@ struct Exec {
     int levels = 0;
     std::vector<uint32_t> level_items;
     std::vector<size_t> level_off;
-    std::vector<char> level_gemm;    // 1: the level runs on the GEMM/SYRK kernel
+    std::vector<char> level_gemm;    // LEVEL_*: the kernel the level runs on
     std::vector<double> level_flops; // algorithmic flops of the units of each level
     const void *level_args[3] = {nullptr, nullptr, nullptr};
 };
