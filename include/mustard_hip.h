@@ -48,6 +48,9 @@ typedef struct {
     int trace;          /* persistent: record per-item wall-clock stamps */
     int small_max;      /* diagonal factorizations above this order nest (0 = 128) */
     int trsm_max;       /* triangular solves above this order nest (0 = 1024) */
+    int lookahead;      /* tiled: 0 (default) = one chain of level kernels; 1 = two
+                           concurrent chains, the trailing-update GEMM/SYRK that feed
+                           only other updates beside everything else (lookahead) */
 } mu_exec_opts_t;
 
 typedef struct {
@@ -60,7 +63,8 @@ typedef struct {
     double critical_path_us;  /* scheduler's estimate */
     int priority_units[4];    /* units per priority level */
     double wall_clock_khz;
-    int levels;               /* tiled mode: kernel launches (DAG levels) per run */
+    int levels;               /* tiled mode: kernel nodes of the graph (launches per run) */
+    int depth;                /* tiled mode: DAG levels (longest chain of units) */
 } mu_exec_stats_t;
 
 /* ---- errors and devices ------------------------------------------------ */
@@ -143,6 +147,7 @@ int mu_dag_set_tag(mu_dag_t dag, int tag);
 #define MU_OP_GETRF 4
 #define MU_OP_POTRF 5
 #define MU_OP_TRINV 6 /* internal: inverted 16x16 diagonal blocks of a large TRSM triangle */
+#define MU_OP_CVT 7   /* internal: a panel tile rounded to the update precision */
 /* Written tile and operation codes of a recorded task (for DOT output and
  * inspection; the reference prints its graphs with cudaGraphDebugDotPrint,
  * lu_mustard.cu:583-585, 634-635). */
@@ -156,20 +161,25 @@ int mu_dag_task_info(mu_dag_t dag, int task, int *w_row, int *w_col, int *ops, i
  * 640).  The recorded device pointers must stay valid. */
 int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t *exec);
 int mu_exec_launch(mu_exec_t exec, void *stream);
-/* Tiled executables: launch the batched kernel of one DAG level alone, and
- * its item count (per-launch timing; a full run is every level in order). */
+/* Tiled executables: launch the batched kernel of one graph node alone, and
+ * its item count (per-launch timing; nodes are numbered in an order that
+ * respects every dependency, so launching 0 .. levels-1 in turn is a full
+ * run without overlap).  The "level" functions below index these nodes. */
 int mu_exec_launch_level(mu_exec_t exec, int level, void *stream);
 int mu_exec_level_items(mu_exec_t exec, int level, long long *items);
-/* Items of one level per operation: counts[MU_OP_*] (7 entries). */
+/* Items of one level per operation: counts[MU_OP_*] (8 entries). */
 int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 /* The kernel one level runs on (MU_KERNEL_GENERIC: k_batch, any item kind;
  * MU_KERNEL_GEMM: k_batch_gemm, GEMM/SYRK blocks only; MU_KERNEL_SMALL:
- * k_batch_small, diagonal GETRF/POTRF of order <= 128 only) and the algorithmic
+ * k_batch_small, diagonal GETRF/POTRF of order <= 128 and precision
+ * conversions only; MU_KERNEL_TRSM: k_batch_trsm, 8-wave panel solves with
+ * inverted diagonal blocks only) and the algorithmic
  * flops of its units (GEMM 2mnk, SYRK n(n+1)k, TRSM m^2 n, GETRF 2m^3/3,
  * POTRF m^3/3).  Either output may be NULL. */
 #define MU_KERNEL_GENERIC 0
 #define MU_KERNEL_GEMM 1
 #define MU_KERNEL_SMALL 2
+#define MU_KERNEL_TRSM 3
 int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);

@@ -175,12 +175,12 @@ class Exec:
     """A lowered, uploaded DAG ready to launch."""
 
     def __init__(self, dag: Dag, mode=MODE_TILED, grid=0, timeout_s=30.0, trace=False,
-                 small_max=0, trsm_max=0):
+                 small_max=0, trsm_max=0, lookahead=False):
         if isinstance(mode, str):
             mode = MODES[mode]
         self.mode = mode
         self.h = ctypes.c_void_p()
-        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max)
+        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, 1 if lookahead else 0)
         call("mu_exec_create", dag.h, mode, ctypes.byref(opts), ctypes.byref(self.h))
 
     def launch(self, stream=None):
@@ -194,16 +194,16 @@ class Exec:
 
     def level_ops(self, level):
         """Items of one level per operation name."""
-        c = (ctypes.c_longlong * 7)()
+        c = (ctypes.c_longlong * 8)()
         call("mu_exec_level_ops", self.h, level, c)
-        names = ["NONE", "GEMM", "SYRK", "TRSM", "GETRF", "POTRF", "TRINV"]
-        return {names[i]: c[i] for i in range(7) if c[i]}
+        names = ["NONE", "GEMM", "SYRK", "TRSM", "GETRF", "POTRF", "TRINV", "CVT"]
+        return {names[i]: c[i] for i in range(8) if c[i]}
 
     def level_info(self, level):
         """(kernel name, algorithmic flops) of one level of a tiled executable."""
         k, f = ctypes.c_int(), ctypes.c_double()
         call("mu_exec_level_info", self.h, level, ctypes.byref(k), ctypes.byref(f))
-        return {1: "k_batch_gemm", 2: "k_batch_small"}.get(k.value, "k_batch"), f.value
+        return {1: "k_batch_gemm", 2: "k_batch_small", 3: "k_batch_trsm"}.get(k.value, "k_batch"), f.value
 
     def level_items(self, level):
         n = ctypes.c_longlong()
@@ -216,7 +216,7 @@ class Exec:
         return {"mode": s.mode, "grid": s.grid, "units": s.units, "items": s.items,
                 "edges": s.edges, "flops": s.flops, "critical_path_us": s.critical_path_us,
                 "priority_units": list(s.priority_units), "wall_clock_khz": s.wall_clock_khz,
-                "levels": s.levels}
+                "levels": s.levels, "depth": s.depth}
 
     def trace(self):
         """Per-item (start, end, unit, task, op) of the last persistent run."""

@@ -26,7 +26,7 @@ class MustardError(RuntimeError):
 class ExecOpts(ctypes.Structure):
     _fields_ = [("grid", ctypes.c_int), ("timeout_s", ctypes.c_double),
                 ("trace", ctypes.c_int), ("small_max", ctypes.c_int),
-                ("trsm_max", ctypes.c_int)]
+                ("trsm_max", ctypes.c_int), ("lookahead", ctypes.c_int)]
 
 
 class ExecStats(ctypes.Structure):
@@ -34,7 +34,7 @@ class ExecStats(ctypes.Structure):
                 ("items", ctypes.c_longlong), ("edges", ctypes.c_longlong),
                 ("flops", ctypes.c_double), ("critical_path_us", ctypes.c_double),
                 ("priority_units", ctypes.c_int * 4), ("wall_clock_khz", ctypes.c_double),
-                ("levels", ctypes.c_int)]
+                ("levels", ctypes.c_int), ("depth", ctypes.c_int)]
 
 
 class IROpts(ctypes.Structure):

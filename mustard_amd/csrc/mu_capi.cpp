@@ -32,6 +32,7 @@ bool emulate(const Dag &dag, const LowerOpts &lo, int threads, double timeout_s,
 
 struct mu_dag_s {
     mu::Dag dag;
+    std::vector<void *> owned;  // device buffers the recorded ops use (freed with the DAG)
 };
 
 namespace mu {
@@ -392,6 +393,8 @@ int mu_dag_destroy(mu_dag_t dag)
             if (kv.second.dag == dag)
                 kv.second.dag = nullptr;
     }
+    for (void *p : dag->owned)
+        (void)hipFree(p);
     delete dag;
     return MU_OK;
 }
@@ -515,12 +518,13 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
             eo.timeout_s = opts->timeout_s;
         eo.grid = opts->grid;
         eo.trace = opts->trace;
+        eo.lookahead = opts->lookahead ? 1 : 0;
     }
     for (const auto &t : dag->dag.tasks)
         for (const auto &op : t.ops)
             if (op.flags & (mu::F_LOW_F32 | mu::F_LOW_BF16))
                 if (op.op != mu::OP_GEMM && op.op != mu::OP_SYRK && op.op != mu::OP_GETRF &&
-                    op.op != mu::OP_POTRF)
+                    op.op != mu::OP_POTRF && op.op != mu::OP_CVT)
                     return fail(MU_ENOTSUP, "reduced precision applies to GEMM/SYRK only");
     mu::Exec *e = nullptr;
     std::string err;
@@ -564,7 +568,7 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts)
     const mu::Exec *e = exec ? exec->e : nullptr;
     if (!e || !counts || level < 0 || level >= e->levels)
         return fail(MU_EINVAL, "mu_exec_level_ops: bad arguments");
-    for (int i = 0; i < 7; i++)
+    for (int i = 0; i < 8; i++)
         counts[i] = 0;
     for (size_t k = e->level_off[level]; k < e->level_off[level + 1]; k++) {
         const uint32_t item = e->level_items[k];
@@ -577,7 +581,7 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts)
             else
                 hi = mid;
         }
-        counts[e->low.units[lo].op % 7]++;
+        counts[e->low.units[lo].op % 8]++;
     }
     return MU_OK;
 }
@@ -590,6 +594,7 @@ int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops)
     if (kernel)
         *kernel = e->level_gemm[level] == mu::LEVEL_GEMM    ? MU_KERNEL_GEMM
                   : e->level_gemm[level] == mu::LEVEL_SMALL ? MU_KERNEL_SMALL
+                  : e->level_gemm[level] == mu::LEVEL_TRSM  ? MU_KERNEL_TRSM
                                                             : MU_KERNEL_GENERIC;
     if (flops)
         *flops = e->level_flops[level];
@@ -621,6 +626,7 @@ int mu_exec_stats(mu_exec_t exec, mu_exec_stats_t *s)
     s->flops = e->low.flops;
     s->critical_path_us = e->low.critical_path;
     s->levels = e->levels;
+    s->depth = e->depth;
     for (const auto &u : e->low.units)
         s->priority_units[u.prio & 3]++;
     s->wall_clock_khz = (double)e->wall_khz;
@@ -687,6 +693,83 @@ int mu_dag_emulate(mu_dag_t dag, int threads, double timeout_s, int bump_unit,
 }
 
 // ---- tiled factorizations -----------------------------------------------------
+// Reduced-precision factorizations keep a shadow of the matrix in the update
+// precision: every panel tile is rounded into it once, right after its TRSM
+// (an OP_CVT op in the same task), and the trailing GEMM/SYRK read their
+// operands from the shadow (F_SRC_LOW) -- half (FP32) or a quarter (BF16) of
+// the FP64 operand traffic.  C stays FP64.
+namespace {
+struct Shadow {
+    char *p = nullptr;
+    size_t elem = 0;
+    int ld = 0, b = 0;
+    const double *at(int i, int j) const
+    {
+        return reinterpret_cast<const double *>(p + ((size_t)i * b + (size_t)j * b * ld) * elem);
+    }
+};
+
+int make_shadow(mu_dag_t dag, int n, int b, int prec, Shadow &sh)
+{
+    if (prec == MU_PREC_FP64)
+        return MU_OK;
+    sh.elem = prec == MU_PREC_FP32 ? 4 : 2;
+    sh.ld = n;
+    sh.b = b;
+    void *p = nullptr;
+    HIPTRY(hipMalloc(&p, (size_t)n * n * sh.elem));
+    dag->owned.push_back(p);
+    sh.p = (char *)p;
+    return MU_OK;
+}
+
+int issue_cvt(void *key, const double *src, int lds, const Shadow &sh, int i, int j, int prec)
+{
+    mu::OpRec op;
+    op.op = mu::OP_CVT;
+    op.flags = prec_flags(prec);
+    op.m = op.n = sh.b;
+    op.c = const_cast<double *>(src);
+    op.ldc = lds;
+    op.b = sh.at(i, j);
+    op.ldb = sh.ld;
+    return issue(key, op);
+}
+
+int issue_gemm_low(void *key, bool tb, const Shadow &sh, int ai, int aj, int bi, int bj,
+                   double *C, int ldc, int prec)
+{
+    mu::OpRec op;
+    op.op = mu::OP_GEMM;
+    op.flags = (tb ? mu::F_TB : 0) | prec_flags(prec) | mu::F_SRC_LOW;
+    op.m = op.n = op.k = sh.b;
+    op.alpha = -1.0;
+    op.beta = 1.0;
+    op.a = sh.at(ai, aj);
+    op.lda = sh.ld;
+    op.b = sh.at(bi, bj);
+    op.ldb = sh.ld;
+    op.c = C;
+    op.ldc = ldc;
+    return issue(key, op);
+}
+
+int issue_syrk_low(void *key, const Shadow &sh, int ai, int aj, double *C, int ldc, int prec)
+{
+    mu::OpRec op;
+    op.op = mu::OP_SYRK;
+    op.flags = prec_flags(prec) | mu::F_SRC_LOW;
+    op.m = op.n = op.k = sh.b;
+    op.alpha = -1.0;
+    op.beta = 1.0;
+    op.a = sh.at(ai, aj);
+    op.lda = sh.ld;
+    op.c = C;
+    op.ldc = ldc;
+    return issue(key, op);
+}
+}  // namespace
+
 int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info_dev, int prec)
 {
     if (!dag || !A || n <= 0 || t <= 0 || n % t != 0 || lda < n)
@@ -694,6 +777,10 @@ int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info
     const int b = n / t;
     void *key = dag;  // record through the same path as a captured stream
     int rc;
+    Shadow sh;
+    if ((rc = make_shadow(dag, n, b, prec, sh)) != 0)
+        return rc;
+    const bool low = sh.p != nullptr;
     if ((rc = mu_dag_capture_begin(dag, key)) != 0)
         return rc;
     auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
@@ -715,6 +802,8 @@ int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info
             rc = mu_dag_begin_task(dag, k, i, r.data(), 2, nullptr);
             if (!rc)
                 rc = mu_dtrsm(key, 'L', 'L', 'N', 'U', b, b, 1.0, blk(k, k), lda, blk(k, i), lda);
+            if (!rc && low)
+                rc = issue_cvt(key, blk(k, i), lda, sh, k, i, prec);
             if (!rc)
                 rc = mu_dag_end_task(dag);
         }
@@ -723,17 +812,18 @@ int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info
             rc = mu_dag_begin_task(dag, i, k, r.data(), 2, nullptr);
             if (!rc)
                 rc = mu_dtrsm(key, 'R', 'U', 'N', 'N', b, b, 1.0, blk(k, k), lda, blk(i, k), lda);
+            if (!rc && low)
+                rc = issue_cvt(key, blk(i, k), lda, sh, i, k, prec);
             if (!rc)
                 rc = mu_dag_end_task(dag);
             for (int j = k + 1; j < t && rc == 0; j++) {
                 auto r3 = rd({i, k, k, j, i, j});
                 rc = mu_dag_begin_task(dag, i, j, r3.data(), 3, nullptr);
-                if (!rc) {
-                    mu_set_update_precision(key, prec);
+                if (!rc && low)
+                    rc = issue_gemm_low(key, false, sh, i, k, k, j, blk(i, j), lda, prec);
+                else if (!rc)
                     rc = mu_dgemm(key, 'N', 'N', b, b, b, -1.0, blk(i, k), lda, blk(k, j), lda,
                                   1.0, blk(i, j), lda);
-                    mu_set_update_precision(key, MU_PREC_FP64);
-                }
                 if (!rc)
                     rc = mu_dag_end_task(dag);
             }
@@ -756,6 +846,10 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
     const int b = n / t;
     void *key = dag;
     int rc;
+    Shadow sh;
+    if ((rc = make_shadow(dag, n, b, prec, sh)) != 0)
+        return rc;
+    const bool low = sh.p != nullptr;
     if ((rc = mu_dag_capture_begin(dag, key)) != 0)
         return rc;
     auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
@@ -776,28 +870,28 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
             rc = mu_dag_begin_task(dag, i, k, r.data(), 2, nullptr);
             if (!rc)
                 rc = mu_dtrsm(key, 'R', 'L', 'T', 'N', b, b, 1.0, blk(k, k), lda, blk(i, k), lda);
+            if (!rc && low)
+                rc = issue_cvt(key, blk(i, k), lda, sh, i, k, prec);
             if (!rc)
                 rc = mu_dag_end_task(dag);
         }
         for (int i = k + 1; i < t && rc == 0; i++) {
             auto r = rd({i, i, i, k});
             rc = mu_dag_begin_task(dag, i, i, r.data(), 2, nullptr);
-            if (!rc) {
-                mu_set_update_precision(key, prec);
+            if (!rc && low)
+                rc = issue_syrk_low(key, sh, i, k, blk(i, i), lda, prec);
+            else if (!rc)
                 rc = mu_dsyrk(key, 'L', 'N', b, b, -1.0, blk(i, k), lda, 1.0, blk(i, i), lda);
-                mu_set_update_precision(key, MU_PREC_FP64);
-            }
             if (!rc)
                 rc = mu_dag_end_task(dag);
             for (int j = i + 1; j < t && rc == 0; j++) {
                 auto r3 = rd({j, i, j, k, i, k});
                 rc = mu_dag_begin_task(dag, j, i, r3.data(), 3, nullptr);
-                if (!rc) {
-                    mu_set_update_precision(key, prec);
+                if (!rc && low)
+                    rc = issue_gemm_low(key, true, sh, j, k, i, k, blk(j, i), lda, prec);
+                else if (!rc)
                     rc = mu_dgemm(key, 'N', 'T', b, b, b, -1.0, blk(j, k), lda, blk(i, k), lda,
                                   1.0, blk(j, i), lda);
-                    mu_set_update_precision(key, MU_PREC_FP64);
-                }
                 if (!rc)
                     rc = mu_dag_end_task(dag);
             }

@@ -85,6 +85,7 @@ int unit_items(const UnitDesc &u)
     case OP_GETRF:
     case OP_POTRF: return 1;
     case OP_TRINV: return u.gm;
+    case OP_CVT: return u.gm * u.gn;
     default: return 0;
     }
 }
@@ -126,7 +127,15 @@ double op_flops(const OpRec &o)
     }
 }
 
-static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+// Operand element pairs are loaded as one vector: 16 bytes for FP64 storage,
+// 8 for FP32, 4 for BF16.
+static bool pair_aligned(const void *p, int flags)
+{
+    const uintptr_t a = (uintptr_t)p;
+    if (!(flags & F_SRC_LOW))
+        return (a & 15) == 0;
+    return (flags & F_LOW_BF16) ? (a & 3) == 0 : (a & 7) == 0;
+}
 
 namespace {
 
@@ -261,8 +270,8 @@ struct Builder {
             UnitDesc d = base(op);
             d.gm = cdiv(op.m, GEMM_BM);
             d.gn = cdiv(op.n, GEMM_BN);
-            if (aligned16(op.a) && aligned16(op.b) && op.lda % 2 == 0 && op.ldb % 2 == 0 &&
-                op.m % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
+            if (pair_aligned(op.a, op.flags) && pair_aligned(op.b, op.flags) && op.lda % 2 == 0 &&
+                op.ldb % 2 == 0 && op.m % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;
             int u = add_unit(d);
             f.src = {u};
@@ -279,8 +288,19 @@ struct Builder {
             d.gm = cdiv(op.n, GEMM_BM);
             d.gn = d.gm;
             d.flags |= F_TB;
-            if (aligned16(op.a) && op.lda % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
+            if (pair_aligned(op.a, op.flags) && op.lda % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;
+            int u = add_unit(d);
+            f.src = {u};
+            f.snk = {u};
+            return true;
+        }
+        case OP_CVT: {
+            if (op.m <= 0 || op.n <= 0)
+                return true;
+            UnitDesc d = base(op);
+            d.gm = cdiv(op.m, GEMM_BM);
+            d.gn = cdiv(op.n, GEMM_BN);
             int u = add_unit(d);
             f.src = {u};
             f.snk = {u};
@@ -555,6 +575,7 @@ double item_cost_us(const UnitDesc &u)
     case OP_GETRF: return 3.0 + 2.0 * u.m * u.m * (double)u.m / 3.0 / (0.05 * per_cu_flops);
     case OP_POTRF: return 3.0 + u.m * u.m * (double)u.m / 3.0 / (0.05 * per_cu_flops);
     case OP_TRINV: return 5.0;
+    case OP_CVT: return 3.0;
     default: return 1.0;
     }
 }

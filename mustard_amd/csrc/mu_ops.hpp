@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mu_types.h"
 
 namespace mu {
@@ -296,25 +298,34 @@ __device__ __forceinline__ unsigned short to_bf16(double x)
 
 template <int P>
 struct LowPrec;
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef unsigned short v2us __attribute__((ext_vector_type(2)));
 template <>
 struct LowPrec<1> {
     using T = float;
+    using T2 = v2f;
     static constexpr int PK = 17;
     static __device__ __forceinline__ float cv(double x) { return (float)x; }
+    static __device__ __forceinline__ float cv(float x) { return x; }
 };
 template <>
 struct LowPrec<2> {
     using T = unsigned short;
+    using T2 = v2us;
     static constexpr int PK = 20;
     static __device__ __forceinline__ unsigned short cv(double x) { return to_bf16(x); }
+    static __device__ __forceinline__ unsigned short cv(unsigned short x) { return x; }
 };
 
-template <bool TA, bool TB, bool VEC, int P>
+template <bool TA, bool TB, bool VEC, int P, bool SRC>
 __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitDesc &u, int bi, int bj,
                                                                      bool lower_only, double *smem)
 {
     using LP = LowPrec<P>;
     using T = typename LP::T;
+    // operand storage: FP64, or (SRC) already rounded to T by an OP_CVT unit
+    using S = typename std::conditional<SRC, T, double>::type;
+    using S2 = typename std::conditional<SRC, typename LP::T2, v2d>::type;
     constexpr int BM = GEMM_BM, BK = 16, PKL = LP::PK, SBUF = BM * PKL;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -322,16 +333,16 @@ __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitD
     const int i0 = bi * BM, j0 = bj * GEMM_BN;
     const int rows = min(BM, u.m - i0), cols = min(GEMM_BN, u.n - j0);
     const int K = u.k;
-    const MU_G double *A = TA ? gp(u.a) + (size_t)i0 * u.lda : gp(u.a) + i0;
-    const MU_G double *B = TB ? gp(u.b) + j0 : gp(u.b) + (size_t)j0 * u.ldb;
-    T *S = reinterpret_cast<T *>(smem);
-    T *As[2] = {S, S + 2 * SBUF};
-    T *Bs[2] = {S + SBUF, S + 3 * SBUF};
+    const MU_G S *A = TA ? (const MU_G S *)u.a + (size_t)i0 * u.lda : (const MU_G S *)u.a + i0;
+    const MU_G S *B = TB ? (const MU_G S *)u.b + j0 : (const MU_G S *)u.b + (size_t)j0 * u.ldb;
+    T *SL = reinterpret_cast<T *>(smem);
+    T *As[2] = {SL, SL + 2 * SBUF};
+    T *Bs[2] = {SL + SBUF, SL + 3 * SBUF};
 
     // One operand block (64 rows x 16 k) per thread: two element pairs.
     // i-contiguous storage (element (r,k) at p[r + k*ld]): pair = rows r, r+1.
     // k-contiguous storage (element (r,k) at p[k + r*ld]): pair = k, k+1.
-    auto ld_op = [&](double (&v)[4], const MU_G double *p, int ld, int nrow, int kc, bool icontig) {
+    auto ld_op = [&](S (&v)[4], const MU_G S *p, int ld, int nrow, int kc, bool icontig) {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int idx = tid + WG_THREADS * q;
@@ -339,8 +350,8 @@ __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitD
                 const int r = (idx & 31) * 2, c = idx >> 5;
                 const int cc = max(0, min(c, kc - 1));
                 if (VEC) {
-                    const v2d x = *reinterpret_cast<const MU_G v2d *>(p + max(0, min(r, nrow - 2)) +
-                                                                    (size_t)cc * ld);
+                    const S2 x = *reinterpret_cast<const MU_G S2 *>(p + max(0, min(r, nrow - 2)) +
+                                                                   (size_t)cc * ld);
                     v[2 * q] = x[0];
                     v[2 * q + 1] = x[1];
                 } else {
@@ -351,8 +362,8 @@ __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitD
                 const int k = (idx & 7) * 2, r = idx >> 3;
                 const int rr = min(r, nrow - 1);
                 if (VEC) {
-                    const v2d x = *reinterpret_cast<const MU_G v2d *>(p + max(0, min(k, kc - 2)) +
-                                                                    (size_t)rr * ld);
+                    const S2 x = *reinterpret_cast<const MU_G S2 *>(p + max(0, min(k, kc - 2)) +
+                                                                   (size_t)rr * ld);
                     v[2 * q] = x[0];
                     v[2 * q + 1] = x[1];
                 } else {
@@ -362,25 +373,26 @@ __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitD
             }
         }
     };
-    auto st_op = [&](T *D, const double (&v)[4], int nrow, int kc, bool icontig) {
+    const T zero = LP::cv(0.0);
+    auto st_op = [&](T *D, const S (&v)[4], int nrow, int kc, bool icontig) {
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int idx = tid + WG_THREADS * q;
             if (icontig) {
                 const int r = (idx & 31) * 2, c = idx >> 5;
                 const bool cin = c < kc;
-                D[r * PKL + c] = LP::cv((cin && r < nrow) ? v[2 * q] : 0.0);
-                D[(r + 1) * PKL + c] = LP::cv((cin && r + 1 < nrow) ? v[2 * q + 1] : 0.0);
+                D[r * PKL + c] = (cin && r < nrow) ? LP::cv(v[2 * q]) : zero;
+                D[(r + 1) * PKL + c] = (cin && r + 1 < nrow) ? LP::cv(v[2 * q + 1]) : zero;
             } else {
                 const int k = (idx & 7) * 2, r = idx >> 3;
                 const bool rin = r < nrow;
-                D[r * PKL + k] = LP::cv((rin && k < kc) ? v[2 * q] : 0.0);
-                D[r * PKL + k + 1] = LP::cv((rin && k + 1 < kc) ? v[2 * q + 1] : 0.0);
+                D[r * PKL + k] = (rin && k < kc) ? LP::cv(v[2 * q]) : zero;
+                D[r * PKL + k + 1] = (rin && k + 1 < kc) ? LP::cv(v[2 * q + 1]) : zero;
             }
         }
     };
-    double ra0[4], rb0[4], ra1[4], rb1[4];
-    auto load = [&](double (&ra)[4], double (&rb)[4], int k0) {
+    S ra0[4], rb0[4], ra1[4], rb1[4];
+    auto load = [&](S (&ra)[4], S (&rb)[4], int k0) {
         const int kc = K - k0, ks = kc > 0 ? k0 : 0;
         if (TA)
             ld_op(ra, A + ks, u.lda, rows, kc, false);
@@ -391,7 +403,7 @@ __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitD
         else
             ld_op(rb, B + ks, u.ldb, cols, kc, false);
     };
-    auto store = [&](const double (&ra)[4], const double (&rb)[4], int buf, int k0) {
+    auto store = [&](const S (&ra)[4], const S (&rb)[4], int buf, int k0) {
         const int kc = K - k0;
         st_op(As[buf], ra, rows, kc, !TA);
         st_op(Bs[buf], rb, cols, kc, TB);
@@ -697,10 +709,12 @@ __device__ __attribute__((always_inline)) inline void trinv_blocks(const UnitDes
 // inverse block and triangle fragments are loaded during the current step
 // (the latter row block by row block, right after that block's MFMAs).
 // ---------------------------------------------------------------------------
-template <bool TT, bool ZR, bool UNIT, bool REV>
+template <bool TT, bool ZR, bool UNIT, bool REV, int NW = 4>
 __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitDesc &u, int panel,
                                                                      double *smem)
 {
+    // NW waves per workgroup (4 in the generic kernel, 8 in k_batch_trsm:
+    // two waves per SIMD hide each other's MFMA and load latency)
     const int n = u.m, p0 = panel * TRSM_W, w = min(TRSM_W, u.n - p0);
     const int np = (n + 15) & ~15, nblk = np >> 4;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -729,7 +743,7 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
         return ZR ? z + (p0 + c) + r * ldz : z + r + (p0 + c) * ldz;
     };
 
-    for (int idx = tid; idx < np * 16; idx += WG_THREADS) {
+    for (int idx = tid; idx < np * 16; idx += 64 * NW) {
         int r, c;
         if (ZR) {
             c = idx & 15;
@@ -742,7 +756,7 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
     }
     lds_barrier();
 
-    constexpr int MAXB = TRSM_NMAX / 64;
+    constexpr int MAXB = TRSM_NMAX / (16 * NW);
     auto load_inv = [&](double (&wv)[4], int jb) {
         const MU_G double *b = winv + (size_t)min(jb >> 4, nblk - 1) * 256 + li * 16 + lk;
 #pragma unroll
@@ -750,7 +764,7 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
             wv[kk] = b[4 * kk];
     };
     auto load_block = [&](double (&f)[4], int jb, int i) {
-        const int rb = jb + 16 + 16 * wave + 64 * i;
+        const int rb = jb + 16 + 16 * wave + 16 * NW * i;
 #pragma unroll
         for (int kk = 0; kk < 4; kk++)
             f[kk] = tv(min(rb + li, n - 1), min(jb + 4 * kk + lk, n - 1));
@@ -781,12 +795,12 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
         // lies past the panel reads clamped rows and is not written back.
 #pragma unroll
         for (int g = 0; g < MAXB / 4; g++) {
-            const int rb0 = jb + 16 + 16 * wave + 256 * g;
+            const int rb0 = jb + 16 + 16 * wave + 64 * NW * g;
             if (rb0 < np) {
                 v4d acc[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int rr = min(rb0 + 64 * q, np - 16);
+                    const int rr = min(rb0 + 16 * NW * q, np - 16);
 #pragma unroll
                     for (int e = 0; e < 4; e++)
                         acc[q][e] = Y[(rr + 4 * e + lk) * 16 + li];
@@ -795,12 +809,12 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
                 for (int kk = 0; kk < 4; kk++)
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        const bool in = rb0 + 64 * q + li < n && jb + 4 * kk + lk < n;
+                        const bool in = rb0 + 16 * NW * q + li < n && jb + 4 * kk + lk < n;
                         acc[q] = mfma_f64(in ? -fa[4 * g + q][kk] : 0.0, xs[kk], acc[q]);
                     }
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int rb = rb0 + 64 * q;
+                    const int rb = rb0 + 16 * NW * q;
                     if (rb < np)
 #pragma unroll
                         for (int e = 0; e < 4; e++)
@@ -1021,16 +1035,27 @@ template <bool TA, bool TB>
 __device__ __forceinline__ void gemm_dispatch_vec(const UnitDesc &u, int bi, int bj, bool lo,
                                                   double *smem)
 {
-    if (u.flags & F_LOW_F32) {
-        if (u.flags & F_VEC)
-            gemm_block_low<TA, TB, true, 1>(u, bi, bj, lo, smem);
-        else
-            gemm_block_low<TA, TB, false, 1>(u, bi, bj, lo, smem);
-    } else if (u.flags & F_LOW_BF16) {
-        if (u.flags & F_VEC)
-            gemm_block_low<TA, TB, true, 2>(u, bi, bj, lo, smem);
-        else
-            gemm_block_low<TA, TB, false, 2>(u, bi, bj, lo, smem);
+    if (u.flags & (F_LOW_F32 | F_LOW_BF16)) {
+#define MU_LOW_CASE(P, SRC)                                               \
+    if (u.flags & F_VEC)                                                  \
+        gemm_block_low<TA, TB, true, P, SRC>(u, bi, bj, lo, smem);        \
+    else                                                                  \
+        gemm_block_low<TA, TB, false, P, SRC>(u, bi, bj, lo, smem);
+        const bool src = (u.flags & F_SRC_LOW) != 0;
+        if (u.flags & F_LOW_F32) {
+            if (src) {
+                MU_LOW_CASE(1, true)
+            } else {
+                MU_LOW_CASE(1, false)
+            }
+        } else {
+            if (src) {
+                MU_LOW_CASE(2, true)
+            } else {
+                MU_LOW_CASE(2, false)
+            }
+        }
+#undef MU_LOW_CASE
     } else if (u.flags & F_VEC) {
         gemm_block<TA, TB, true>(u, bi, bj, lo, smem);
     } else {
@@ -1057,14 +1082,56 @@ __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDe
     }
 }
 
-// Small GETRF/POTRF item (the only kinds k_batch_small runs).
-__device__ __attribute__((always_inline)) inline void run_small_item(const UnitDesc &u,
+// FP64 block -> FP32 / BF16 storage: item = one 64 x 64 block of the m x n
+// source (c, ldc), written to (b, ldb).
+__device__ __attribute__((always_inline)) inline void cvt_block(const UnitDesc &u, int it)
+{
+    const int bi = it % u.gm, bj = it / u.gm;
+    const int i0 = bi * 64, j0 = bj * 64;
+    const MU_G double *src = gp(u.c);
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int idx = threadIdx.x + WG_THREADS * q, r = i0 + (idx & 63), c = j0 + (idx >> 6);
+        if (r < u.m && c < u.n) {
+            const double v = src[r + (size_t)c * u.ldc];
+            if (u.flags & F_LOW_BF16)
+                ((MU_G unsigned short *)u.b)[r + (size_t)c * u.ldb] = to_bf16(v);
+            else
+                ((MU_G float *)u.b)[r + (size_t)c * u.ldb] = (float)v;
+        }
+    }
+}
+
+// TRSM item with inverted diagonal blocks on NW waves (k_batch_trsm).
+template <int NW>
+__device__ __attribute__((always_inline)) inline void run_trsm_inv_item(const UnitDesc &u, int it,
+                                                                        double *smem)
+{
+    switch ((u.flags >> 3) & 0x27) {  // T_TRANS, Z_ROWS, UNIT, REV
+#define MU_TRSM_INV_CASE(bits)                                                                 \
+    case bits:                                                                                 \
+        trsm_panel_inv<(bits & 1) != 0, (bits & 2) != 0, (bits & 4) != 0, (bits & 0x20) != 0, \
+                       NW>(u, it, smem);                                                       \
+        break;
+        MU_TRSM_INV_CASE(0x00) MU_TRSM_INV_CASE(0x01) MU_TRSM_INV_CASE(0x02) MU_TRSM_INV_CASE(0x03)
+        MU_TRSM_INV_CASE(0x04) MU_TRSM_INV_CASE(0x05) MU_TRSM_INV_CASE(0x06) MU_TRSM_INV_CASE(0x07)
+        MU_TRSM_INV_CASE(0x20) MU_TRSM_INV_CASE(0x21) MU_TRSM_INV_CASE(0x22) MU_TRSM_INV_CASE(0x23)
+        MU_TRSM_INV_CASE(0x24) MU_TRSM_INV_CASE(0x25) MU_TRSM_INV_CASE(0x26) MU_TRSM_INV_CASE(0x27)
+#undef MU_TRSM_INV_CASE
+    default: break;
+    }
+}
+
+// Small GETRF/POTRF or CVT item (the only kinds k_batch_small runs).
+__device__ __attribute__((always_inline)) inline void run_small_item(const UnitDesc &u, int it,
                                                                      double *smem)
 {
     if (u.op == OP_GETRF)
         getrf_small(u, smem);
-    else
+    else if (u.op == OP_POTRF)
         potrf_small(u, smem);
+    else
+        cvt_block(u, it);
 }
 
 // Execute local item `it` of unit `u`.  Always inlined, so that LDS stays
@@ -1096,6 +1163,7 @@ __device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u
         break;
     case OP_GETRF: getrf_small(u, smem); break;
     case OP_POTRF: potrf_small(u, smem); break;
+    case OP_CVT: cvt_block(u, it); break;
     case OP_TRINV:
         switch ((u.flags >> 3) & 0x25) {  // T_TRANS, UNIT, REV
         case 0x00: trinv_blocks<false, false, false>(u, it, smem); break;

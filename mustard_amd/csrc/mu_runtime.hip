@@ -79,11 +79,22 @@ __global__ __launch_bounds__(WG_THREADS) void k_batch_small(const UnitDesc *unit
     __shared__ double smem[LDS_SMALL];
     const uint32_t item = items[blockIdx.x];
     const UnitDesc d = units[item_unit[item]];
-    run_small_item(d, smem);
+    run_small_item(d, (int)(item - d.item_base), smem);
+}
+
+constexpr int TRSM_KERNEL_WAVES = 8;
+__global__ __launch_bounds__(64 * TRSM_KERNEL_WAVES) void k_batch_trsm(const UnitDesc *units,
+                                                                      const uint32_t *item_unit,
+                                                                      const uint32_t *items)
+{
+    __shared__ double smem[TRSM_NMAX * 16];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    run_trsm_inv_item<TRSM_KERNEL_WAVES>(d, (int)(item - d.item_base), smem);
 }
 
 static bool gemm_only_op(int op) { return op == OP_GEMM || op == OP_SYRK; }
-static bool small_only_op(int op) { return op == OP_GETRF || op == OP_POTRF; }
+static bool small_only_op(int op) { return op == OP_GETRF || op == OP_POTRF || op == OP_CVT; }
 
 // Algorithmic flops of one lowered unit (the whole operation its items cover).
 static double unit_flops(const UnitDesc &u)
@@ -292,9 +303,15 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
             item_unit[L.units[u].item_base + i] = u;
 
     if (mode == MODE_GRAPH) {
-        // Level of each unit = longest chain of units before it.  Units of one
-        // level are independent, so one kernel runs all their items; the
-        // levels form a chain of kernel nodes in one hipGraph.
+        // Level of each unit = longest chain of units before it; units of one
+        // level are independent.  Lane: a GEMM/SYRK unit whose successors are
+        // all GEMM/SYRK units (a trailing update that feeds only further
+        // updates) goes to the bulk lane 1, every other unit to lane 0.  Each
+        // (lane, level) with work is one kernel node; the nodes of a lane form
+        // a chain, and a unit whose predecessor sits in the other lane adds an
+        // edge between the two nodes.  Levels increase along every unit edge,
+        // so the node graph is acyclic, and the bulk update of one step runs
+        // beside the next step's panel work (lookahead).
         std::vector<int> level(nu, 0);
         int nlev = nu ? 1 : 0;
         for (uint32_t u = 0; u < nu; u++)
@@ -302,10 +319,54 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                 level[v] = std::max(level[v], level[u] + 1);
                 nlev = std::max(nlev, level[v] + 1);
             }
-        std::vector<std::vector<uint32_t>> lv(nlev);
+        std::vector<int> lane(nu, 0);
+        if (eopts.lookahead)
+            for (uint32_t u = 0; u < nu; u++) {
+                if (!gemm_only_op(L.units[u].op))
+                    continue;
+                bool bulk = true;
+                for (uint32_t v : L.succ[u])
+                    bulk = bulk && gemm_only_op(L.units[v].op);
+                lane[u] = bulk ? 1 : 0;
+            }
+        // nodes in (level, lane) order: a topological order of the node graph
+        std::vector<int> node_of(2 * (size_t)std::max(nlev, 1), -1);
+        std::vector<std::vector<uint32_t>> lv;
+        std::vector<int> node_lane;
+        {
+            std::vector<std::vector<uint32_t>> by(2 * (size_t)std::max(nlev, 1));
+            for (uint32_t u = 0; u < nu; u++)
+                for (int i = 0; i < L.units[u].nitems; i++)
+                    by[2 * (size_t)level[u] + lane[u]].push_back(L.units[u].item_base + i);
+            for (size_t k = 0; k < by.size(); k++)
+                if (!by[k].empty()) {
+                    node_of[k] = (int)lv.size();
+                    node_lane.push_back((int)(k & 1));
+                    lv.push_back(std::move(by[k]));
+                }
+        }
+        const int nnode = (int)lv.size();
+        // node of a unit; units without items (none are lowered) map to -1
+        auto unode = [&](uint32_t u) { return node_of[2 * (size_t)level[u] + lane[u]]; };
+        std::vector<std::vector<int>> node_deps(nnode);
+        int last[2] = {-1, -1};
+        for (int k = 0; k < nnode; k++) {
+            const int ln = node_lane[k];
+            if (last[ln] >= 0)
+                node_deps[k].push_back(last[ln]);
+            last[ln] = k;
+        }
         for (uint32_t u = 0; u < nu; u++)
-            for (int i = 0; i < L.units[u].nitems; i++)
-                lv[level[u]].push_back(L.units[u].item_base + i);
+            for (uint32_t v : L.succ[u])
+                if (lane[u] != lane[v]) {
+                    const int a = unode(u), b = unode(v);
+                    if (a >= 0 && b >= 0 && a != b)
+                        node_deps[b].push_back(a);
+                }
+        for (auto &d : node_deps) {
+            std::sort(d.begin(), d.end());
+            d.erase(std::unique(d.begin(), d.end()), d.end());
+        }
         std::vector<uint32_t> items;
         std::vector<size_t> off;
         for (auto &v : lv) {
@@ -335,49 +396,55 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         const UnitDesc *du = reinterpret_cast<const UnitDesc *>(e->dbuf + o_units);
         const uint32_t *diu = reinterpret_cast<const uint32_t *>(e->dbuf + o_iu);
         const uint32_t *ditems = reinterpret_cast<const uint32_t *>(e->dbuf + o_items);
-        e->level_flops.assign(nlev, 0.0);
+        e->level_flops.assign(nnode, 0.0);
         for (uint32_t u = 0; u < nu; u++)
             if (L.units[u].nitems > 0)
-                e->level_flops[level[u]] += unit_flops(L.units[u]);
-        // a level whose items are all GEMM/SYRK blocks runs on k_batch_gemm,
-        // one of only small diagonal factorizations on k_batch_small (both
-        // with small LDS, so several workgroups share a CU)
-        e->level_gemm.assign(nlev, LEVEL_GENERIC);
-        for (int l = 0; l < nlev; l++) {
-            bool g = true, sm = true;
+                e->level_flops[unode(u)] += unit_flops(L.units[u]);
+        // a node whose items are all GEMM/SYRK blocks runs on k_batch_gemm,
+        // one of only small diagonal factorizations / conversions on
+        // k_batch_small (both with small LDS, so several workgroups share a CU)
+        // a node of only TRSM items with inverted diagonal blocks runs on
+        // k_batch_trsm (8 waves per workgroup)
+        e->level_gemm.assign(nnode, LEVEL_GENERIC);
+        for (int l = 0; l < nnode; l++) {
+            bool g = true, sm = true, tr = true;
             for (uint32_t it : lv[l]) {
-                const int op = L.units[item_unit[it]].op;
-                g = g && gemm_only_op(op);
-                sm = sm && small_only_op(op);
+                const UnitDesc &d = L.units[item_unit[it]];
+                g = g && gemm_only_op(d.op);
+                sm = sm && small_only_op(d.op);
+                tr = tr && d.op == OP_TRSM && (d.flags & F_INV);
             }
-            e->level_gemm[l] = g ? LEVEL_GEMM : sm ? LEVEL_SMALL : LEVEL_GENERIC;
+            e->level_gemm[l] = g ? LEVEL_GEMM : sm ? LEVEL_SMALL : tr ? LEVEL_TRSM : LEVEL_GENERIC;
         }
         HIPCHK(hipGraphCreate(&e->graph, 0));
-        hipGraphNode_t prev = nullptr;
-        for (int l = 0; l < nlev; l++) {
+        std::vector<hipGraphNode_t> gnodes(nnode);
+        for (int l = 0; l < nnode; l++) {
             const uint32_t *first = ditems + off[l];
             void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
             hipKernelNodeParams kp;
             std::memset(&kp, 0, sizeof kp);
             kp.func = e->level_gemm[l] == LEVEL_GEMM    ? reinterpret_cast<void *>(k_batch_gemm)
                       : e->level_gemm[l] == LEVEL_SMALL ? reinterpret_cast<void *>(k_batch_small)
+                      : e->level_gemm[l] == LEVEL_TRSM  ? reinterpret_cast<void *>(k_batch_trsm)
                                                         : reinterpret_cast<void *>(k_batch);
             kp.gridDim = dim3((unsigned)lv[l].size());
-            kp.blockDim = dim3(WG_THREADS);
+            kp.blockDim = dim3(e->level_gemm[l] == LEVEL_TRSM ? 64 * TRSM_KERNEL_WAVES : WG_THREADS);
             kp.kernelParams = args;
-            hipGraphNode_t node;
-            HIPCHK(hipGraphAddKernelNode(&node, e->graph, prev ? &prev : nullptr, prev ? 1 : 0,
-                                         &kp));
-            prev = node;
+            std::vector<hipGraphNode_t> deps;
+            for (int d : node_deps[l])
+                deps.push_back(gnodes[d]);
+            HIPCHK(hipGraphAddKernelNode(&gnodes[l], e->graph, deps.empty() ? nullptr : deps.data(),
+                                         deps.size(), &kp));
         }
-        e->levels = nlev;
+        e->levels = nnode;
+        e->depth = nlev;
         e->level_items = std::move(items);
         e->level_off.assign(off.begin(), off.end());
         e->level_off.push_back(e->level_items.size());
         e->level_args[0] = (const void *)du;
         e->level_args[1] = (const void *)diu;
         e->level_args[2] = (const void *)ditems;
-        if (nlev)
+        if (nnode)
             HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
         *out = e;
         return 0;
@@ -509,6 +576,9 @@ int exec_launch_level(Exec *e, int level, hipStream_t s, std::string &err)
         hipLaunchKernelGGL(k_batch_gemm, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
     else if (e->level_gemm[level] == LEVEL_SMALL)
         hipLaunchKernelGGL(k_batch_small, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
+    else if (e->level_gemm[level] == LEVEL_TRSM)
+        hipLaunchKernelGGL(k_batch_trsm, dim3(n), dim3(64 * TRSM_KERNEL_WAVES), 0, s, du, diu,
+                           first);
     else
         hipLaunchKernelGGL(k_batch, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
     HIPCHK(hipGetLastError());

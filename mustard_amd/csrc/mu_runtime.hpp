@@ -19,9 +19,10 @@ struct ExecOpts {
     int grid = 0;              // persistent grid; 0 = one workgroup per resident slot
     double timeout_s = 30.0;   // persistent: seconds without progress before abort
     int trace = 0;             // persistent: record per-item wall-clock stamps
+    int lookahead = 0;         // tiled: bulk updates on a second, concurrent chain
 };
 
-enum LevelKernel : char { LEVEL_GENERIC = 0, LEVEL_GEMM = 1, LEVEL_SMALL = 2 };
+enum LevelKernel : char { LEVEL_GENERIC = 0, LEVEL_GEMM = 1, LEVEL_SMALL = 2, LEVEL_TRSM = 3 };
 
 struct Exec {
     int mode = MODE_PERSISTENT;
@@ -41,7 +42,8 @@ struct Exec {
     // graph mode
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int levels = 0;
+    int levels = 0;                  // graph nodes (level kernels of both chains)
+    int depth = 0;                   // DAG levels
     std::vector<uint32_t> level_items;
     std::vector<size_t> level_off;
     std::vector<char> level_gemm;    // LEVEL_*: the kernel the level runs on

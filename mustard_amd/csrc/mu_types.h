@@ -22,6 +22,7 @@ enum Op : int32_t {
     OP_POTRF = 5,  // lower Cholesky of an n <= SMALL_MAX tile in LDS; one item
     OP_TRINV = 6,  // inverses of the 16 x 16 diagonal blocks of a TRSM triangle, into the
                    // unit's workspace slot; items = groups of TRINV_BLOCKS blocks
+    OP_CVT = 7,    // FP64 block (c, ldc) rounded to FP32 / BF16 storage (b, ldb); items = 64 x 64
 };
 
 // Unit flags.
@@ -36,6 +37,7 @@ enum Flag : int32_t {
     F_LOW_BF16 = 1 << 7, // GEMM/SYRK: operands rounded to BF16, FP32 accumulation
     F_REV = 1 << 8,      // TRSM: row order of T and Z reversed (upper-triangular solves)
     F_INV = 1 << 9,      // TRSM: diagonal blocks solved with the inverses a TRINV unit left in b
+    F_SRC_LOW = 1 << 10, // GEMM/SYRK with F_LOW_*: a and b are stored in that precision (not FP64)
 };
 
 // Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).

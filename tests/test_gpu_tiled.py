@@ -184,4 +184,27 @@ def test_level_batching_stats(torch_gpu, oracle):
     dA = mu.to_device(oracle.gen_ref(n))
     dag = mu.Dag().record_lu(mu.ptr(dA), n, t)
     ex = mu.Exec(dag, "tiled")
-    assert ex.stats()["levels"] == 3 * (t - 1) + 1
+    st = ex.stats()
+    assert st["levels"] == st["depth"] == 3 * (t - 1) + 1
+    # with lookahead the bulk updates of steps 0 .. t-3 (those not feeding the
+    # next panel) get nodes of their own on the second chain
+    la = mu.Exec(dag, "tiled", lookahead=True).stats()
+    assert la["depth"] == st["depth"] and la["levels"] == st["depth"] + (t - 2)
+
+
+@pytest.mark.parametrize("kind", ["lu", "chol"])
+@pytest.mark.parametrize("lookahead", [True, False])
+def test_lookahead_graph_matches_oracle(torch_gpu, oracle, kind, lookahead):
+    """The two-chain (lookahead) graph and the single chain give the oracle's
+    factors, and the same bits as each other."""
+    mu = _mu()
+    n, t = 2000, 8
+    A = oracle.gen_ref(n)
+    ref = A.copy(order="F")
+    assert (oracle.tiled_lu if kind == "lu" else oracle.tiled_cholesky)(ref, t) == 0
+    dA = mu.to_device(A)
+    info, st = mu.factorize(kind, dA, n, t, lookahead=lookahead)
+    F = mu.to_host(dA)
+    m = np.tril(np.ones((n, n), dtype=bool)) if kind == "chol" else np.ones((n, n), dtype=bool)
+    err = np.max(np.abs(F[m] - ref[m]) / np.maximum(np.abs(ref[m]), 1e-3 * np.abs(ref).max()))
+    assert info == 0 and err <= 1e-10
