@@ -971,6 +971,23 @@ int sym_avg(const double *src, int lds, double *dst, int ldd, int n, hipStream_t
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// t consecutive b x b identity tiles (tile k at D + k b^2, leading dimension b)
+__global__ void k_identity_tiles(double *D, int b, int t)
+{
+    const size_t total = (size_t)b * b * t;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t e = k % ((size_t)b * b);
+        D[k] = (e % b) == (e / b) ? 1.0 : 0.0;
+    }
+}
+
+int identity_tiles(double *D, int b, int t, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_identity_tiles, dim3(2048), dim3(256), 0, s, D, b, t);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int gen_cond_start(double *A, int n, int lda, double kappa, hipStream_t s)
 {
     hipLaunchKernelGGL(k_diag_geometric, dim3(4096), dim3(256), 0, s, A, (size_t)lda, n, kappa);

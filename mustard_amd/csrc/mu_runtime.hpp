@@ -84,6 +84,8 @@ int inf_norm(const double *A, int lda, int m, int n, double *out, hipStream_t s)
 int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipStream_t s);
 // conditioned test matrices: A = diag(kappa^(-i/(n-1))); acc += x.y; A <- H A H
 int gen_cond_start(double *A, int n, int lda, double kappa, hipStream_t s);
+// t consecutive b x b identity tiles (tile k at D + k b^2, leading dimension b)
+int identity_tiles(double *D, int b, int t, hipStream_t s);
 // dst = tril(src) with the diagonal times dscale (zeros above); dst = (src + src^T) / 2
 int tril_scaled(const double *src, int lds, double *dst, int ldd, int n, double dscale,
                 hipStream_t s);

@@ -10,7 +10,9 @@
 // driver when asked to.  Here the factors are kept in FP64 storage: only the
 // trailing GEMM/SYRK of the factorization run in FP32 or BF16 (the diagonal
 // factorizations, the panel solves, the triangular solves of the refinement,
-// the residual and the correction are FP64).
+// the residual and the correction are FP64).  The refinement's triangular
+// solves use the inverted diagonal tiles of the factors (computed once per
+// factorization), so each substitution step is a GEMM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -104,35 +106,104 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     const int t = opts->tiles, itermax = opts->itermax > 0 ? opts->itermax : 30;
     *res = mu_ir_result_t{};
 
-    // workspace: factors F (n x n), right-hand side / correction R (n x nrhs),
-    // the info cell and the norms
-    DevBuf dF, dR, dN, dI;
+    // workspace: factors F (n x n), right-hand side / correction R and the
+    // forward-substitution result Y (n x nrhs each), the inverted diagonal
+    // tiles of the factors (t tiles of b x b for L, and for U with LU), the
+    // info cell and the norms
+    const int b = n / t;
+    const int ninv = opts->kind == MU_SOLVE_LU ? 2 : 1;
+    DevBuf dF, dR, dY, dD, dN, dI;
     HK(hipMalloc(&dF.p, (size_t)n * n * sizeof(double)));
     HK(hipMalloc(&dR.p, (size_t)n * nrhs * sizeof(double)));
+    HK(hipMalloc(&dY.p, (size_t)n * nrhs * sizeof(double)));
+    HK(hipMalloc(&dD.p, (size_t)ninv * n * b * sizeof(double)));
     HK(hipMalloc(&dN.p, (size_t)(2 * nrhs + 1) * sizeof(double)));
     HK(hipMalloc(&dI.p, sizeof(int)));
-    double *F = (double *)dF.p, *R = (double *)dR.p;
+    double *F = (double *)dF.p, *R = (double *)dR.p, *Y = (double *)dY.p;
+    double *Linv = (double *)dD.p, *Uinv = Linv + (size_t)n * b;
     double *rn = (double *)dN.p, *xn = rn + nrhs, *an = xn + nrhs;
     int *info = (int *)dI.p;
 
 
-    // factorization and solve graphs over F and R
-    auto build = [&](int prec, Dag &fd, Ex &fx, Dag &sd, Ex &sx) -> int {
+    // Graphs over F, R, Y: the factorization (with the inversion of its
+    // diagonal tiles, D_k = T_kk^-1 by a TRSM against the identity, once per
+    // factorization) and the solve.  With the inverses every step of the
+    // substitutions is a GEMM over all of its output blocks instead of a
+    // single-workgroup panel solve:
+    //   forward   Y_k = Linv_k R_k;          R_i -= L_ik Y_k      (i > k)
+    //   backward  R_k = Uinv_k Y_k (LU)  or  Linv_k^T Y_k (Cholesky);
+    //             Y_i -= U_ik R_k  or  L_ki^T R_k                 (i < k)
+    // leaving the solution in R.
+    auto fb = [&](int i, int j) { return F + (size_t)i * b + (size_t)j * b * n; };
+    auto dinv = [&](double *D, int k) { return D + (size_t)k * b * b; };
+    auto build = [&](int prec, Dag &fd, Ex &fx, Dag &id, Ex &ix, Dag &sd, Ex &sx) -> int {
         CK(mu_dag_create(&fd.h));
         if (opts->kind == MU_SOLVE_LU)
             CK(mu_record_tiled_lu(fd.h, F, n, t, n, info, prec));
         else
             CK(mu_record_tiled_cholesky(fd.h, F, n, t, n, info, prec));
         CK(mu_exec_create(fd.h, MU_MODE_TILED, nullptr, &fx.h));
+        // inverses of the diagonal tiles (independent tasks)
+        CK(mu_dag_create(&id.h));
+        void *key = id.h;
+        CK(mu_dag_capture_begin(id.h, key));
+        for (int k = 0; k < t; k++) {
+            CK(mu_dag_begin_task(id.h, k, 0, nullptr, 0, nullptr));
+            CK(mu_dtrsm(key, 'L', 'L', 'N', opts->kind == MU_SOLVE_LU ? 'U' : 'N', b, b, 1.0,
+                        fb(k, k), n, dinv(Linv, k), b));
+            CK(mu_dag_end_task(id.h));
+            if (opts->kind == MU_SOLVE_LU) {
+                CK(mu_dag_begin_task(id.h, k, 1, nullptr, 0, nullptr));
+                CK(mu_dtrsm(key, 'L', 'U', 'N', 'N', b, b, 1.0, fb(k, k), n, dinv(Uinv, k), b));
+                CK(mu_dag_end_task(id.h));
+            }
+        }
+        CK(mu_dag_capture_end(id.h, key));
+        CK(mu_exec_create(id.h, MU_MODE_TILED, nullptr, &ix.h));
+        // the solve: tile k of R is (k, 0), of Y is (k, 1)
         CK(mu_dag_create(&sd.h));
-        CK(mu_record_tiled_solve(sd.h, F, n, t, n, opts->kind, R, n, nrhs));
+        key = sd.h;
+        CK(mu_dag_capture_begin(sd.h, key));
+        auto rb = [&](double *M, int k) { return M + (size_t)k * b; };
+        for (int k = 0; k < t; k++) {
+            int rd0[2] = {k, 0};
+            CK(mu_dag_begin_task(sd.h, k, 1, rd0, 1, nullptr));
+            CK(mu_dgemm(key, 'N', 'N', b, nrhs, b, 1.0, dinv(Linv, k), b, rb(R, k), n, 0.0,
+                        rb(Y, k), n));
+            CK(mu_dag_end_task(sd.h));
+            for (int i = k + 1; i < t; i++) {
+                int rd[4] = {k, 1, i, 0};
+                CK(mu_dag_begin_task(sd.h, i, 0, rd, 2, nullptr));
+                CK(mu_dgemm(key, 'N', 'N', b, nrhs, b, -1.0, fb(i, k), n, rb(Y, k), n, 1.0,
+                            rb(R, i), n));
+                CK(mu_dag_end_task(sd.h));
+            }
+        }
+        const bool lu = opts->kind == MU_SOLVE_LU;
+        for (int k = t - 1; k >= 0; k--) {
+            int rd0[2] = {k, 1};
+            CK(mu_dag_begin_task(sd.h, k, 0, rd0, 1, nullptr));
+            CK(mu_dgemm(key, lu ? 'N' : 'T', 'N', b, nrhs, b, 1.0, dinv(lu ? Uinv : Linv, k), b,
+                        rb(Y, k), n, 0.0, rb(R, k), n));
+            CK(mu_dag_end_task(sd.h));
+            for (int i = 0; i < k; i++) {
+                int rd[4] = {k, 0, i, 1};
+                CK(mu_dag_begin_task(sd.h, i, 1, rd, 2, nullptr));
+                CK(mu_dgemm(key, lu ? 'N' : 'T', 'N', b, nrhs, b, -1.0, lu ? fb(i, k) : fb(k, i), n,
+                            rb(R, k), n, 1.0, rb(Y, i), n));
+                CK(mu_dag_end_task(sd.h));
+            }
+        }
+        CK(mu_dag_capture_end(sd.h, key));
         CK(mu_exec_create(sd.h, MU_MODE_TILED, nullptr, &sx.h));
         return 0;
     };
-    auto factor = [&](Ex &fx) -> int {
+    auto factor = [&](Ex &fx, Ex &ix) -> int {
         KK(mu::copy_matrix(A, lda, F, n, n, n, s));
         CK(mu_info_reset(info, s));
         CK(mu_exec_launch(fx.h, s));
+        KK(mu::identity_tiles(Linv, b, t * ninv, s));
+        CK(mu_exec_launch(ix.h, s));
         HK(hipMemcpyAsync(&res->info, info, sizeof(int), hipMemcpyDeviceToHost, s));
         HK(hipStreamSynchronize(s));
         if (res->info == INT32_MAX)
@@ -153,9 +224,9 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     };
 
     // graphs are built (lowered, uploaded, instantiated) before the clock starts
-    Dag fd, sd;
-    Ex fx, sx;
-    CK(build(opts->prec, fd, fx, sd, sx));
+    Dag fd, idg, sd;
+    Ex fx, ix, sx;
+    CK(build(opts->prec, fd, fx, idg, ix, sd, sx));
     Ev e0, e1, e2;
     HK(hipEventCreate(&e0.e));
     HK(hipEventCreate(&e1.e));
@@ -183,7 +254,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
         return true;
     };
 
-    CK(factor(fx));
+    CK(factor(fx, ix));
     HK(hipEventRecord(e1.e, s));
     if (res->info != 0) {
         mu_solve_ir_impl_set_error("mu_solve_ir: the factorization failed (info > 0)");
@@ -215,10 +286,10 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     res->iters = it;
     if (!res->converged && opts->fallback && opts->prec != MU_PREC_FP64) {
         // LAPACK's fallback: refactor in full precision and solve
-        Dag fd2, sd2;
-        Ex fx2, sx2;
-        CK(build(MU_PREC_FP64, fd2, fx2, sd2, sx2));
-        CK(factor(fx2));
+        Dag fd2, id2, sd2;
+        Ex fx2, ix2, sx2;
+        CK(build(MU_PREC_FP64, fd2, fx2, id2, ix2, sd2, sx2));
+        CK(factor(fx2, ix2));
         if (res->info != 0) {
             mu_solve_ir_impl_set_error("mu_solve_ir: the FP64 factorization failed (info > 0)");
             return MU_ERUN;
