@@ -48,9 +48,10 @@ typedef struct {
     int trace;          /* persistent: record per-item wall-clock stamps */
     int small_max;      /* diagonal factorizations above this order nest (0 = 128) */
     int trsm_max;       /* triangular solves above this order nest (0 = 1024) */
-    int lookahead;      /* tiled: 0 (default) = one chain of level kernels; 1 = two
-                           concurrent chains, the trailing-update GEMM/SYRK that feed
-                           only other updates beside everything else (lookahead) */
+    int lookahead;      /* tiled: 0 (default) or 1 = two concurrent chains of level
+                           kernels, the trailing-update GEMM/SYRK tasks that feed only
+                           other updates beside everything else (lookahead);
+                           -1 = one chain */
 } mu_exec_opts_t;
 
 typedef struct {
@@ -173,13 +174,15 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
  * MU_KERNEL_GEMM: k_batch_gemm, GEMM/SYRK blocks only; MU_KERNEL_SMALL:
  * k_batch_small, diagonal GETRF/POTRF of order <= 128 and precision
  * conversions only; MU_KERNEL_TRSM: k_batch_trsm, 8-wave panel solves with
- * inverted diagonal blocks only) and the algorithmic
+ * inverted diagonal blocks only; MU_KERNEL_TRSM_SMALL: k_batch_trsm_small,
+ * panel solves of order <= 128 without inverted blocks only) and the algorithmic
  * flops of its units (GEMM 2mnk, SYRK n(n+1)k, TRSM m^2 n, GETRF 2m^3/3,
  * POTRF m^3/3).  Either output may be NULL. */
 #define MU_KERNEL_GENERIC 0
 #define MU_KERNEL_GEMM 1
 #define MU_KERNEL_SMALL 2
 #define MU_KERNEL_TRSM 3
+#define MU_KERNEL_TRSM_SMALL 4
 int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);

@@ -175,12 +175,12 @@ class Exec:
     """A lowered, uploaded DAG ready to launch."""
 
     def __init__(self, dag: Dag, mode=MODE_TILED, grid=0, timeout_s=30.0, trace=False,
-                 small_max=0, trsm_max=0, lookahead=False):
+                 small_max=0, trsm_max=0, lookahead=True):
         if isinstance(mode, str):
             mode = MODES[mode]
         self.mode = mode
         self.h = ctypes.c_void_p()
-        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, 1 if lookahead else 0)
+        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, 1 if lookahead else -1)
         call("mu_exec_create", dag.h, mode, ctypes.byref(opts), ctypes.byref(self.h))
 
     def launch(self, stream=None):
@@ -203,7 +203,8 @@ class Exec:
         """(kernel name, algorithmic flops) of one level of a tiled executable."""
         k, f = ctypes.c_int(), ctypes.c_double()
         call("mu_exec_level_info", self.h, level, ctypes.byref(k), ctypes.byref(f))
-        return {1: "k_batch_gemm", 2: "k_batch_small", 3: "k_batch_trsm"}.get(k.value, "k_batch"), f.value
+        return {1: "k_batch_gemm", 2: "k_batch_small", 3: "k_batch_trsm",
+                4: "k_batch_trsm_small"}.get(k.value, "k_batch"), f.value
 
     def level_items(self, level):
         n = ctypes.c_longlong()

@@ -518,7 +518,7 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
             eo.timeout_s = opts->timeout_s;
         eo.grid = opts->grid;
         eo.trace = opts->trace;
-        eo.lookahead = opts->lookahead ? 1 : 0;
+        eo.lookahead = opts->lookahead >= 0 ? 1 : 0;
     }
     for (const auto &t : dag->dag.tasks)
         for (const auto &op : t.ops)
@@ -595,7 +595,8 @@ int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops)
         *kernel = e->level_gemm[level] == mu::LEVEL_GEMM    ? MU_KERNEL_GEMM
                   : e->level_gemm[level] == mu::LEVEL_SMALL ? MU_KERNEL_SMALL
                   : e->level_gemm[level] == mu::LEVEL_TRSM  ? MU_KERNEL_TRSM
-                                                            : MU_KERNEL_GENERIC;
+                  : e->level_gemm[level] == mu::LEVEL_TRSM_SMALL ? MU_KERNEL_TRSM_SMALL
+                                                                 : MU_KERNEL_GENERIC;
     if (flops)
         *flops = e->level_flops[level];
     return MU_OK;

@@ -495,14 +495,16 @@ __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitD
 // Z is scaled by alpha on load.  The panel lives in LDS; each 16-row step solves its diagonal block with one
 // thread per right-hand side and updates the rows below with FP64 MFMA.
 // ---------------------------------------------------------------------------
-template <bool TT, bool ZR, bool UNIT, bool REV>
+template <bool TT, bool ZR, bool UNIT, bool REV, int NMAX = TRSM_NMAX>
 __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc &u, int panel, double *smem)
 {
+    // NMAX: largest triangle the caller's LDS holds (TRSM_NMAX in the generic
+    // kernel, SMALL_MAX in k_batch_trsm_small)
     const int n = u.m, p0 = panel * TRSM_W, w = min(TRSM_W, u.n - p0);
     const int np = (n + 15) & ~15;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double *Y = smem;                   // [np][16]
-    double *Td = smem + TRSM_NMAX * 16; // [16][17]
+    double *Y = smem;              // [np][16]
+    double *Td = smem + NMAX * 16; // [16][17]
     const MU_G double *t = gp(u.a);
     const size_t ldt = u.lda, ldz = u.ldc;
     MU_G double *z = gp(u.c);
@@ -548,7 +550,7 @@ __device__ __attribute__((always_inline)) inline void trsm_panel(const UnitDesc 
     // in-order vector-memory counter tracks 63).
     const int li = lane & 15, lk = lane >> 4;
     const int r16 = tid >> 4, c16 = tid & 15;
-    constexpr int MAXB = TRSM_NMAX / 64;
+    constexpr int MAXB = NMAX / 64 > 0 ? NMAX / 64 : 1;
     // Loads are unconditional, from addresses clamped into the triangle;
     // values outside it are zeroed where they are used (see the header of
     // the GEMM loaders for why no load sits behind a branch).
@@ -1121,6 +1123,29 @@ __device__ __attribute__((always_inline)) inline void run_trsm_inv_item(const Un
     default: break;
     }
 }
+
+// Panel solve of a triangle of order <= SMALL_MAX without inverted blocks
+// (k_batch_trsm_small: the panel in 18.5 KB of LDS).
+__device__ __attribute__((always_inline)) inline void run_trsm_small_item(const UnitDesc &u, int it,
+                                                                          double *smem)
+{
+    switch ((u.flags >> 3) & 0x27) {  // T_TRANS, Z_ROWS, UNIT, REV
+#define MU_TRSM_SMALL_CASE(bits)                                                           \
+    case bits:                                                                             \
+        trsm_panel<(bits & 1) != 0, (bits & 2) != 0, (bits & 4) != 0, (bits & 0x20) != 0, \
+                   SMALL_MAX>(u, it, smem);                                                \
+        break;
+        MU_TRSM_SMALL_CASE(0x00) MU_TRSM_SMALL_CASE(0x01) MU_TRSM_SMALL_CASE(0x02)
+        MU_TRSM_SMALL_CASE(0x03) MU_TRSM_SMALL_CASE(0x04) MU_TRSM_SMALL_CASE(0x05)
+        MU_TRSM_SMALL_CASE(0x06) MU_TRSM_SMALL_CASE(0x07) MU_TRSM_SMALL_CASE(0x20)
+        MU_TRSM_SMALL_CASE(0x21) MU_TRSM_SMALL_CASE(0x22) MU_TRSM_SMALL_CASE(0x23)
+        MU_TRSM_SMALL_CASE(0x24) MU_TRSM_SMALL_CASE(0x25) MU_TRSM_SMALL_CASE(0x26)
+        MU_TRSM_SMALL_CASE(0x27)
+#undef MU_TRSM_SMALL_CASE
+    default: break;
+    }
+}
+constexpr int LDS_TRSM_SMALL = SMALL_MAX * 16 + 16 * 17;
 
 // Small GETRF/POTRF or CVT item (the only kinds k_batch_small runs).
 __device__ __attribute__((always_inline)) inline void run_small_item(const UnitDesc &u, int it,

@@ -82,6 +82,17 @@ __global__ __launch_bounds__(WG_THREADS) void k_batch_small(const UnitDesc *unit
     run_small_item(d, (int)(item - d.item_base), smem);
 }
 
+// Panel solves of order <= SMALL_MAX (inside nested diagonal factorizations).
+__global__ __launch_bounds__(WG_THREADS) void k_batch_trsm_small(const UnitDesc *units,
+                                                                 const uint32_t *item_unit,
+                                                                 const uint32_t *items)
+{
+    __shared__ double smem[LDS_TRSM_SMALL];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    run_trsm_small_item(d, (int)(item - d.item_base), smem);
+}
+
 constexpr int TRSM_KERNEL_WAVES = 8;
 __global__ __launch_bounds__(64 * TRSM_KERNEL_WAVES) void k_batch_trsm(const UnitDesc *units,
                                                                       const uint32_t *item_unit,
@@ -94,6 +105,17 @@ __global__ __launch_bounds__(64 * TRSM_KERNEL_WAVES) void k_batch_trsm(const Uni
 }
 
 static bool gemm_only_op(int op) { return op == OP_GEMM || op == OP_SYRK; }
+
+static const void *level_kernel(int kind)
+{
+    switch (kind) {
+    case LEVEL_GEMM: return reinterpret_cast<const void *>(k_batch_gemm);
+    case LEVEL_SMALL: return reinterpret_cast<const void *>(k_batch_small);
+    case LEVEL_TRSM: return reinterpret_cast<const void *>(k_batch_trsm);
+    case LEVEL_TRSM_SMALL: return reinterpret_cast<const void *>(k_batch_trsm_small);
+    default: return reinterpret_cast<const void *>(k_batch);
+    }
+}
 static bool small_only_op(int op) { return op == OP_GETRF || op == OP_POTRF || op == OP_CVT; }
 
 // Algorithmic flops of one lowered unit (the whole operation its items cover).
@@ -319,14 +341,25 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                 level[v] = std::max(level[v], level[u] + 1);
                 nlev = std::max(nlev, level[v] + 1);
             }
+        // a unit belongs to a trailing update when its recorded task is a
+        // GEMM/SYRK task (not a piece of a nested factorization or solve)
+        auto update_task = [&](uint32_t u) {
+            const int tk = L.units[u].task;
+            if (tk < 0 || tk >= (int)dag.tasks.size() || dag.tasks[tk].ops.empty())
+                return false;
+            for (const OpRec &op : dag.tasks[tk].ops)
+                if (!gemm_only_op(op.op))
+                    return false;
+            return true;
+        };
         std::vector<int> lane(nu, 0);
         if (eopts.lookahead)
             for (uint32_t u = 0; u < nu; u++) {
-                if (!gemm_only_op(L.units[u].op))
+                if (!update_task(u))
                     continue;
                 bool bulk = true;
                 for (uint32_t v : L.succ[u])
-                    bulk = bulk && gemm_only_op(L.units[v].op);
+                    bulk = bulk && update_task(v);
                 lane[u] = bulk ? 1 : 0;
             }
         // nodes in (level, lane) order: a topological order of the node graph
@@ -405,16 +438,22 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         // k_batch_small (both with small LDS, so several workgroups share a CU)
         // a node of only TRSM items with inverted diagonal blocks runs on
         // k_batch_trsm (8 waves per workgroup)
+        // ... and one of small panel solves (order <= 128) on k_batch_trsm_small
         e->level_gemm.assign(nnode, LEVEL_GENERIC);
         for (int l = 0; l < nnode; l++) {
-            bool g = true, sm = true, tr = true;
+            bool g = true, sm = true, tr = true, ts = true;
             for (uint32_t it : lv[l]) {
                 const UnitDesc &d = L.units[item_unit[it]];
                 g = g && gemm_only_op(d.op);
                 sm = sm && small_only_op(d.op);
                 tr = tr && d.op == OP_TRSM && (d.flags & F_INV);
+                ts = ts && d.op == OP_TRSM && !(d.flags & F_INV) && d.m <= SMALL_MAX;
             }
-            e->level_gemm[l] = g ? LEVEL_GEMM : sm ? LEVEL_SMALL : tr ? LEVEL_TRSM : LEVEL_GENERIC;
+            e->level_gemm[l] = g    ? LEVEL_GEMM
+                               : sm ? LEVEL_SMALL
+                               : tr ? LEVEL_TRSM
+                               : ts ? LEVEL_TRSM_SMALL
+                                    : LEVEL_GENERIC;
         }
         HIPCHK(hipGraphCreate(&e->graph, 0));
         std::vector<hipGraphNode_t> gnodes(nnode);
@@ -423,10 +462,7 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
             void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
             hipKernelNodeParams kp;
             std::memset(&kp, 0, sizeof kp);
-            kp.func = e->level_gemm[l] == LEVEL_GEMM    ? reinterpret_cast<void *>(k_batch_gemm)
-                      : e->level_gemm[l] == LEVEL_SMALL ? reinterpret_cast<void *>(k_batch_small)
-                      : e->level_gemm[l] == LEVEL_TRSM  ? reinterpret_cast<void *>(k_batch_trsm)
-                                                        : reinterpret_cast<void *>(k_batch);
+            kp.func = const_cast<void *>(level_kernel(e->level_gemm[l]));
             kp.gridDim = dim3((unsigned)lv[l].size());
             kp.blockDim = dim3(e->level_gemm[l] == LEVEL_TRSM ? 64 * TRSM_KERNEL_WAVES : WG_THREADS);
             kp.kernelParams = args;
@@ -572,15 +608,11 @@ int exec_launch_level(Exec *e, int level, hipStream_t s, std::string &err)
     const uint32_t *diu = (const uint32_t *)e->level_args[1];
     const uint32_t *first = (const uint32_t *)e->level_args[2] + e->level_off[level];
     const unsigned n = (unsigned)(e->level_off[level + 1] - e->level_off[level]);
-    if (e->level_gemm[level] == LEVEL_GEMM)
-        hipLaunchKernelGGL(k_batch_gemm, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
-    else if (e->level_gemm[level] == LEVEL_SMALL)
-        hipLaunchKernelGGL(k_batch_small, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
-    else if (e->level_gemm[level] == LEVEL_TRSM)
-        hipLaunchKernelGGL(k_batch_trsm, dim3(n), dim3(64 * TRSM_KERNEL_WAVES), 0, s, du, diu,
-                           first);
-    else
-        hipLaunchKernelGGL(k_batch, dim3(n), dim3(WG_THREADS), 0, s, du, diu, first);
+    void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
+    HIPCHK(hipLaunchKernel(level_kernel(e->level_gemm[level]), dim3(n),
+                           dim3(e->level_gemm[level] == LEVEL_TRSM ? 64 * TRSM_KERNEL_WAVES
+                                                                   : WG_THREADS),
+                           args, 0, s));
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -19,10 +19,16 @@ struct ExecOpts {
     int grid = 0;              // persistent grid; 0 = one workgroup per resident slot
     double timeout_s = 30.0;   // persistent: seconds without progress before abort
     int trace = 0;             // persistent: record per-item wall-clock stamps
-    int lookahead = 0;         // tiled: bulk updates on a second, concurrent chain
+    int lookahead = 1;         // tiled: bulk updates on a second, concurrent chain
 };
 
-enum LevelKernel : char { LEVEL_GENERIC = 0, LEVEL_GEMM = 1, LEVEL_SMALL = 2, LEVEL_TRSM = 3 };
+enum LevelKernel : char {
+    LEVEL_GENERIC = 0,
+    LEVEL_GEMM = 1,
+    LEVEL_SMALL = 2,
+    LEVEL_TRSM = 3,
+    LEVEL_TRSM_SMALL = 4,
+};
 
 struct Exec {
     int mode = MODE_PERSISTENT;

@@ -183,12 +183,12 @@ def test_level_batching_stats(torch_gpu, oracle):
     n, t = 1024, 8
     dA = mu.to_device(oracle.gen_ref(n))
     dag = mu.Dag().record_lu(mu.ptr(dA), n, t)
-    ex = mu.Exec(dag, "tiled")
+    ex = mu.Exec(dag, "tiled", lookahead=False)
     st = ex.stats()
     assert st["levels"] == st["depth"] == 3 * (t - 1) + 1
-    # with lookahead the bulk updates of steps 0 .. t-3 (those not feeding the
-    # next panel) get nodes of their own on the second chain
-    la = mu.Exec(dag, "tiled", lookahead=True).stats()
+    # with lookahead (the default) the bulk updates of steps 0 .. t-3 (those
+    # not feeding the next panel) get nodes of their own on the second chain
+    la = mu.Exec(dag, "tiled").stats()
     assert la["depth"] == st["depth"] and la["levels"] == st["depth"] + (t - 2)
 
 
