@@ -47,7 +47,7 @@ typedef struct {
     double timeout_s;   /* persistent: abort after this long without progress (0 = 30 s) */
     int trace;          /* persistent: record per-item wall-clock stamps */
     int small_max;      /* diagonal factorizations above this order nest (0 = 128) */
-    int trsm_max;       /* triangular solves above this order nest (0 = 1024) */
+    int trsm_max;       /* triangular solves above this order nest (0 = 256; at most 1024) */
     int lookahead;      /* tiled: 0 (default) or 1 = two concurrent chains of level
                            kernels, the trailing-update GEMM/SYRK tasks that feed only
                            other updates beside everything else (lookahead);
@@ -175,7 +175,10 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
  * k_batch_small, diagonal GETRF/POTRF of order <= 128 and precision
  * conversions only; MU_KERNEL_TRSM: k_batch_trsm, 8-wave panel solves with
  * inverted diagonal blocks only; MU_KERNEL_TRSM_SMALL: k_batch_trsm_small,
- * panel solves of order <= 128 without inverted blocks only) and the algorithmic
+ * panel solves of order <= 128 without inverted blocks only; MU_KERNEL_TRSM_256 /
+ * MU_KERNEL_TRSM_512: k_batch_trsm_mid<256 / 512>, 4-wave panel solves with
+ * inverted diagonal blocks of order <= 256 / 512 and the block inversions
+ * before them) and the algorithmic
  * flops of its units (GEMM 2mnk, SYRK n(n+1)k, TRSM m^2 n, GETRF 2m^3/3,
  * POTRF m^3/3).  Either output may be NULL. */
 #define MU_KERNEL_GENERIC 0
@@ -183,6 +186,8 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 #define MU_KERNEL_SMALL 2
 #define MU_KERNEL_TRSM 3
 #define MU_KERNEL_TRSM_SMALL 4
+#define MU_KERNEL_TRSM_256 5
+#define MU_KERNEL_TRSM_512 6
 int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);

@@ -141,6 +141,10 @@ namespace {
 
 struct Frag {
     std::vector<int> src, snk;
+    // sources that read only the triangle of a solve (TRINV units): inside a
+    // nested sub-DAG they wait for the triangle's producer, not the
+    // right-hand side's
+    std::vector<int> tri;
 };
 
 struct Builder {
@@ -200,8 +204,10 @@ struct Builder {
         std::vector<Frag> frags;
         std::vector<int> is_pred;
         explicit SubRec(Builder &bb) : b(bb) {}
+        // tri: key of the block holding a solve's triangle ({-1, -1}: not
+        // written inside this sub-DAG)
         bool add(std::pair<int, int> w, std::initializer_list<std::pair<int, int>> r,
-                 const OpRec &op)
+                 const OpRec &op, std::pair<int, int> tri = {-1, -1})
         {
             Frag f;
             if (!b.lower_op(op, f))
@@ -215,10 +221,21 @@ struct Builder {
             for (auto key : r)
                 look(key);
             look(w);
+            auto is_tri = [&](int u) {
+                return std::find(f.tri.begin(), f.tri.end(), u) != f.tri.end();
+            };
             for (const Frag *p : preds)
                 for (int s : p->snk)
                     for (int d : f.src)
-                        b.edge(s, d);
+                        if (!is_tri(d))
+                            b.edge(s, d);
+            if (!f.tri.empty() && tri.first >= 0) {
+                auto it = last.find(tri);
+                if (it != last.end())
+                    for (int s : it->second.snk)
+                        for (int d : f.tri)
+                            b.edge(s, d);
+            }
             last[w] = f;
             frags.push_back(f);
             return true;
@@ -335,8 +352,9 @@ struct Builder {
                     const int ut = add_unit(t);
                     const int u = add_unit(d);
                     edge(ut, u);
-                    f.src = {ut};
+                    f.src = {ut, u};
                     f.snk = {u};
+                    f.tri = {ut};
                     return true;
                 }
                 int u = add_unit(d);
@@ -395,7 +413,7 @@ struct Builder {
                 t.c = blk(k, i);
                 t.ldc = (int)ld;
                 t.alpha = 1.0;
-                if (!rec.add({k, i}, {{k, k}, {k, i}}, t))
+                if (!rec.add({k, i}, {{k, k}, {k, i}}, t, {k, k}))
                     return false;
             }
             for (int i = k + 1; i < P; i++) {
@@ -409,7 +427,7 @@ struct Builder {
                 t.c = blk(i, k);
                 t.ldc = (int)ld;
                 t.alpha = 1.0;
-                if (!rec.add({i, k}, {{k, k}, {i, k}}, t))
+                if (!rec.add({i, k}, {{k, k}, {i, k}}, t, {k, k}))
                     return false;
                 for (int j = k + 1; j < P; j++) {
                     OpRec g2;
@@ -464,7 +482,7 @@ struct Builder {
                 t.c = blk(i, k);
                 t.ldc = (int)ld;
                 t.alpha = 1.0;
-                if (!rec.add({i, k}, {{k, k}, {i, k}}, t))
+                if (!rec.add({i, k}, {{k, k}, {i, k}}, t, {k, k}))
                     return false;
             }
             for (int i = k + 1; i < P; i++) {

@@ -64,7 +64,7 @@ class Dag {
 
 struct LowerOpts {
     int small_max = SMALL_MAX;  // diagonal factorizations above this nest into sub-tiles
-    int trsm_max = TRSM_NMAX;   // triangular solves above this nest into blocks
+    int trsm_max = TRSM_NEST;   // triangular solves above this nest into blocks (<= TRSM_NMAX)
     int trsm_inv_min = SMALL_MAX + 1;  // triangles of at least this order use inverted diagonal blocks
     int nprio = NPRIO;
 };

@@ -711,12 +711,14 @@ __device__ __attribute__((always_inline)) inline void trinv_blocks(const UnitDes
 // inverse block and triangle fragments are loaded during the current step
 // (the latter row block by row block, right after that block's MFMAs).
 // ---------------------------------------------------------------------------
-template <bool TT, bool ZR, bool UNIT, bool REV, int NW = 4>
+template <bool TT, bool ZR, bool UNIT, bool REV, int NW = 4, int NMAX = TRSM_NMAX>
 __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitDesc &u, int panel,
                                                                      double *smem)
 {
     // NW waves per workgroup (4 in the generic kernel, 8 in k_batch_trsm:
-    // two waves per SIMD hide each other's MFMA and load latency)
+    // two waves per SIMD hide each other's MFMA and load latency); NMAX the
+    // largest triangle the caller's LDS holds
+    static_assert(NMAX % (64 * NW) == 0, "row blocks come in groups of four per wave");
     const int n = u.m, p0 = panel * TRSM_W, w = min(TRSM_W, u.n - p0);
     const int np = (n + 15) & ~15, nblk = np >> 4;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -758,7 +760,7 @@ __device__ __attribute__((always_inline)) inline void trsm_panel_inv(const UnitD
     }
     lds_barrier();
 
-    constexpr int MAXB = TRSM_NMAX / (16 * NW);
+    constexpr int MAXB = NMAX / (16 * NW);
     auto load_inv = [&](double (&wv)[4], int jb) {
         const MU_G double *b = winv + (size_t)min(jb >> 4, nblk - 1) * 256 + li * 16 + lk;
 #pragma unroll
@@ -1104,8 +1106,9 @@ __device__ __attribute__((always_inline)) inline void cvt_block(const UnitDesc &
     }
 }
 
-// TRSM item with inverted diagonal blocks on NW waves (k_batch_trsm).
-template <int NW>
+// TRSM item with inverted diagonal blocks on NW waves (k_batch_trsm,
+// k_batch_trsm_mid).
+template <int NW, int NMAX = TRSM_NMAX>
 __device__ __attribute__((always_inline)) inline void run_trsm_inv_item(const UnitDesc &u, int it,
                                                                         double *smem)
 {
@@ -1113,7 +1116,7 @@ __device__ __attribute__((always_inline)) inline void run_trsm_inv_item(const Un
 #define MU_TRSM_INV_CASE(bits)                                                                 \
     case bits:                                                                                 \
         trsm_panel_inv<(bits & 1) != 0, (bits & 2) != 0, (bits & 4) != 0, (bits & 0x20) != 0, \
-                       NW>(u, it, smem);                                                       \
+                       NW, NMAX>(u, it, smem);                                                 \
         break;
         MU_TRSM_INV_CASE(0x00) MU_TRSM_INV_CASE(0x01) MU_TRSM_INV_CASE(0x02) MU_TRSM_INV_CASE(0x03)
         MU_TRSM_INV_CASE(0x04) MU_TRSM_INV_CASE(0x05) MU_TRSM_INV_CASE(0x06) MU_TRSM_INV_CASE(0x07)
@@ -1121,6 +1124,22 @@ __device__ __attribute__((always_inline)) inline void run_trsm_inv_item(const Un
         MU_TRSM_INV_CASE(0x24) MU_TRSM_INV_CASE(0x25) MU_TRSM_INV_CASE(0x26) MU_TRSM_INV_CASE(0x27)
 #undef MU_TRSM_INV_CASE
     default: break;
+    }
+}
+
+// TRINV item (inverses of TRINV_BLOCKS diagonal blocks).
+__device__ __attribute__((always_inline)) inline void run_trinv_item(const UnitDesc &u, int it,
+                                                                     double *smem)
+{
+    switch ((u.flags >> 3) & 0x25) {  // T_TRANS, UNIT, REV
+    case 0x00: trinv_blocks<false, false, false>(u, it, smem); break;
+    case 0x01: trinv_blocks<true, false, false>(u, it, smem); break;
+    case 0x04: trinv_blocks<false, true, false>(u, it, smem); break;
+    case 0x05: trinv_blocks<true, true, false>(u, it, smem); break;
+    case 0x20: trinv_blocks<false, false, true>(u, it, smem); break;
+    case 0x21: trinv_blocks<true, false, true>(u, it, smem); break;
+    case 0x24: trinv_blocks<false, true, true>(u, it, smem); break;
+    default: trinv_blocks<true, true, true>(u, it, smem); break;
     }
 }
 
@@ -1189,18 +1208,7 @@ __device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u
     case OP_GETRF: getrf_small(u, smem); break;
     case OP_POTRF: potrf_small(u, smem); break;
     case OP_CVT: cvt_block(u, it); break;
-    case OP_TRINV:
-        switch ((u.flags >> 3) & 0x25) {  // T_TRANS, UNIT, REV
-        case 0x00: trinv_blocks<false, false, false>(u, it, smem); break;
-        case 0x01: trinv_blocks<true, false, false>(u, it, smem); break;
-        case 0x04: trinv_blocks<false, true, false>(u, it, smem); break;
-        case 0x05: trinv_blocks<true, true, false>(u, it, smem); break;
-        case 0x20: trinv_blocks<false, false, true>(u, it, smem); break;
-        case 0x21: trinv_blocks<true, false, true>(u, it, smem); break;
-        case 0x24: trinv_blocks<false, true, true>(u, it, smem); break;
-        default: trinv_blocks<true, true, true>(u, it, smem); break;
-        }
-        break;
+    case OP_TRINV: run_trinv_item(u, it, smem); break;
     default: break;
     }
 }

@@ -104,6 +104,25 @@ __global__ __launch_bounds__(64 * TRSM_KERNEL_WAVES) void k_batch_trsm(const Uni
     run_trsm_inv_item<TRSM_KERNEL_WAVES>(d, (int)(item - d.item_base), smem);
 }
 
+// Panel solves with inverted diagonal blocks of order <= NMAX, and the TRINV
+// items before them, on 4 waves and at most NMAX * 128 bytes of LDS
+// (k_batch_trsm_256: 34 KB, k_batch_trsm_512: 64 KB), so GEMM workgroups keep
+// running beside them on the same CU.
+template <int NMAX>
+__global__ __launch_bounds__(WG_THREADS) void k_batch_trsm_mid(const UnitDesc *units,
+                                                               const uint32_t *item_unit,
+                                                               const uint32_t *items)
+{
+    constexpr int LDSD = NMAX * 16 > TRINV_BLOCKS * 16 * 17 ? NMAX * 16 : TRINV_BLOCKS * 16 * 17;
+    __shared__ double smem[LDSD];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    if (d.op == OP_TRINV)
+        run_trinv_item(d, (int)(item - d.item_base), smem);
+    else
+        run_trsm_inv_item<4, NMAX>(d, (int)(item - d.item_base), smem);
+}
+
 static bool gemm_only_op(int op) { return op == OP_GEMM || op == OP_SYRK; }
 
 static const void *level_kernel(int kind)
@@ -113,6 +132,8 @@ static const void *level_kernel(int kind)
     case LEVEL_SMALL: return reinterpret_cast<const void *>(k_batch_small);
     case LEVEL_TRSM: return reinterpret_cast<const void *>(k_batch_trsm);
     case LEVEL_TRSM_SMALL: return reinterpret_cast<const void *>(k_batch_trsm_small);
+    case LEVEL_TRSM_256: return reinterpret_cast<const void *>(k_batch_trsm_mid<256>);
+    case LEVEL_TRSM_512: return reinterpret_cast<const void *>(k_batch_trsm_mid<512>);
     default: return reinterpret_cast<const void *>(k_batch);
     }
 }
@@ -441,16 +462,21 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         // ... and one of small panel solves (order <= 128) on k_batch_trsm_small
         e->level_gemm.assign(nnode, LEVEL_GENERIC);
         for (int l = 0; l < nnode; l++) {
-            bool g = true, sm = true, tr = true, ts = true;
+            bool g = true, sm = true, tr = true, ts = true, tm = true;
+            int tmax = 0;  // largest triangle of the level's TRSM / TRINV items
             for (uint32_t it : lv[l]) {
                 const UnitDesc &d = L.units[item_unit[it]];
                 g = g && gemm_only_op(d.op);
                 sm = sm && small_only_op(d.op);
                 tr = tr && d.op == OP_TRSM && (d.flags & F_INV);
                 ts = ts && d.op == OP_TRSM && !(d.flags & F_INV) && d.m <= SMALL_MAX;
+                tm = tm && (d.op == OP_TRINV || (d.op == OP_TRSM && (d.flags & F_INV)));
+                tmax = std::max(tmax, (int)d.m);
             }
+            tm = tm && tmax <= 512;
             e->level_gemm[l] = g    ? LEVEL_GEMM
                                : sm ? LEVEL_SMALL
+                               : tm ? (tmax <= 256 ? LEVEL_TRSM_256 : LEVEL_TRSM_512)
                                : tr ? LEVEL_TRSM
                                : ts ? LEVEL_TRSM_SMALL
                                     : LEVEL_GENERIC;

@@ -28,6 +28,8 @@ enum LevelKernel : char {
     LEVEL_SMALL = 2,
     LEVEL_TRSM = 3,
     LEVEL_TRSM_SMALL = 4,
+    LEVEL_TRSM_256 = 5,
+    LEVEL_TRSM_512 = 6,
 };
 
 struct Exec {

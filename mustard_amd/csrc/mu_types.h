@@ -47,6 +47,8 @@ constexpr int GEMM_BN = 64;     // ... and columns
 constexpr int GEMM_BK = 8;      // k-block staged in LDS per step
 constexpr int TRSM_W = 16;        // right-hand sides per TRSM item
 constexpr int TRSM_NMAX = 1024;   // largest triangle a TRSM item keeps in LDS
+constexpr int TRSM_NEST = 256;    // default: triangles above this order nest into blocks,
+                                  // whose solves run beside GEMM workgroups on a CU
 constexpr int SMALL_MAX = 128;    // largest tile a GETRF/POTRF item factors in LDS
 constexpr int NPRIO = 4;          // priority levels of the device queue
 constexpr int TRINV_BLOCKS = 16;  // diagonal blocks one TRINV item inverts (16 threads each)

@@ -208,3 +208,29 @@ def test_lookahead_graph_matches_oracle(torch_gpu, oracle, kind, lookahead):
     m = np.tril(np.ones((n, n), dtype=bool)) if kind == "chol" else np.ones((n, n), dtype=bool)
     err = np.max(np.abs(F[m] - ref[m]) / np.maximum(np.abs(ref[m]), 1e-3 * np.abs(ref).max()))
     assert info == 0 and err <= 1e-10
+
+
+@pytest.mark.parametrize("kind", ["lu", "chol"])
+@pytest.mark.parametrize("trsm_max,kernel", [(0, "k_batch_trsm_256"), (512, "k_batch_trsm_512"),
+                                             (1024, "k_batch_trsm")])
+def test_trsm_nesting_threshold(torch_gpu, oracle, kind, trsm_max, kernel):
+    """Panel solves of 1000-tiles nest into blocks of at most trsm_max (256 by
+    default) and run on the kernel sized for that block; every threshold gives
+    the oracle's factors."""
+    mu = _mu()
+    n, t = 3000, 3
+    A = oracle.gen_ref(n)
+    ref = A.copy(order="F")
+    assert (oracle.tiled_lu if kind == "lu" else oracle.tiled_cholesky)(ref, t) == 0
+    dA = mu.to_device(A)
+    dag = mu.Dag()
+    (dag.record_lu if kind == "lu" else dag.record_cholesky)(mu.ptr(dA), n, t)
+    ex = mu.Exec(dag, "tiled", trsm_max=trsm_max)
+    kernels = {ex.level_info(lv)[0] for lv in range(ex.stats()["levels"])}
+    assert kernel in kernels
+    ex.launch()
+    torch_gpu.cuda.synchronize()
+    F = mu.to_host(dA)
+    m = np.tril(np.ones((n, n), dtype=bool)) if kind == "chol" else np.ones((n, n), dtype=bool)
+    err = np.max(np.abs(F[m] - ref[m]) / np.maximum(np.abs(ref[m]), 1e-3 * np.abs(ref).max()))
+    assert err <= 1e-10
