@@ -347,14 +347,11 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
 
     if (mode == MODE_GRAPH) {
         // Level of each unit = longest chain of units before it; units of one
-        // level are independent.  Lane: a GEMM/SYRK unit whose successors are
-        // all GEMM/SYRK units (a trailing update that feeds only further
-        // updates) goes to the bulk lane 1, every other unit to lane 0.  Each
-        // (lane, level) with work is one kernel node; the nodes of a lane form
-        // a chain, and a unit whose predecessor sits in the other lane adds an
-        // edge between the two nodes.  Levels increase along every unit edge,
-        // so the node graph is acyclic, and the bulk update of one step runs
-        // beside the next step's panel work (lookahead).
+        // level are independent.  Lane: a unit of a trailing update that
+        // feeds only further updates goes to the bulk lane 1, every unit on
+        // the way to the next panel to lane 0.  Levels increase along every
+        // unit edge, so the node graph below is acyclic, and the bulk update
+        // of one step runs beside the next step's panel work (lookahead).
         std::vector<int> level(nu, 0);
         int nlev = nu ? 1 : 0;
         for (uint32_t u = 0; u < nu; u++)
@@ -363,7 +360,8 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                 nlev = std::max(nlev, level[v] + 1);
             }
         // a unit belongs to a trailing update when its recorded task is a
-        // GEMM/SYRK task (not a piece of a nested factorization or solve)
+        // GEMM/SYRK task (not a piece of a nested factorization or solve);
+        // such a unit whose successors are all update units is bulk work ...
         auto update_task = [&](uint32_t u) {
             const int tk = L.units[u].task;
             if (tk < 0 || tk >= (int)dag.tasks.size() || dag.tasks[tk].ops.empty())
@@ -373,42 +371,94 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                     return false;
             return true;
         };
+        // ... and so does every unit of a task no other task depends on (a
+        // result nothing in this graph reads, such as the inverse of a
+        // factored diagonal tile) whose successors stay inside that task
+        std::vector<char> sink_task(dag.tasks.size(), 1);
+        for (const TaskRec &tr : dag.tasks)
+            for (int d : tr.deps)
+                if (d >= 0 && d < (int)sink_task.size())
+                    sink_task[d] = 0;
         std::vector<int> lane(nu, 0);
         if (eopts.lookahead)
             for (uint32_t u = 0; u < nu; u++) {
-                if (!update_task(u))
-                    continue;
-                bool bulk = true;
-                for (uint32_t v : L.succ[u])
-                    bulk = bulk && update_task(v);
-                lane[u] = bulk ? 1 : 0;
+                const int tk = L.units[u].task;
+                if (update_task(u)) {
+                    bool bulk = true;
+                    for (uint32_t v : L.succ[u])
+                        bulk = bulk && update_task(v);
+                    lane[u] = bulk ? 1 : 0;
+                } else if (tk >= 0 && tk < (int)sink_task.size() && sink_task[tk]) {
+                    bool inner = true;
+                    for (uint32_t v : L.succ[u])
+                        inner = inner && L.units[v].task == tk;
+                    lane[u] = inner ? 1 : 0;
+                }
             }
-        // nodes in (level, lane) order: a topological order of the node graph
-        std::vector<int> node_of(2 * (size_t)std::max(nlev, 1), -1);
+        // the kernel a unit's items run on: GEMM/SYRK blocks on k_batch_gemm,
+        // small diagonal factorizations / conversions on k_batch_small, panel
+        // solves with inverted diagonal blocks (and the inversions, TRINV) on
+        // the 4-wave k_batch_trsm_mid<256 / 512> or, for larger triangles, the
+        // 8-wave k_batch_trsm, small solves without inverses on
+        // k_batch_trsm_small; anything else on the generic k_batch
+        auto ukind = [&](const UnitDesc &d) -> int {
+            if (gemm_only_op(d.op))
+                return LEVEL_GEMM;
+            if (small_only_op(d.op))
+                return LEVEL_SMALL;
+            if (d.op == OP_TRINV)
+                return LEVEL_TRSM_256;
+            if (d.op == OP_TRSM && (d.flags & F_INV))
+                return d.m <= 256 ? LEVEL_TRSM_256 : d.m <= 512 ? LEVEL_TRSM_512 : LEVEL_TRSM;
+            if (d.op == OP_TRSM && d.m <= SMALL_MAX)
+                return LEVEL_TRSM_SMALL;
+            return LEVEL_GENERIC;
+        };
+        // one kernel node per (level, lane, kernel) with work, in that order
+        // (a topological order of the node graph).  The nodes of a lane form a
+        // chain of levels: each depends on every node of the lane's previous
+        // level; a unit whose predecessor sits in the other lane adds an edge
+        // between the two nodes.
+        const size_t NK = LEVEL_KINDS;
+        const size_t nkey = 2 * NK * (size_t)std::max(nlev, 1);
+        auto ukey = [&](uint32_t u) {
+            return (2 * (size_t)level[u] + lane[u]) * NK + ukind(L.units[u]);
+        };
+        std::vector<int> node_of(nkey, -1);
         std::vector<std::vector<uint32_t>> lv;
-        std::vector<int> node_lane;
+        std::vector<int> node_lane, node_level;
         {
-            std::vector<std::vector<uint32_t>> by(2 * (size_t)std::max(nlev, 1));
+            std::vector<std::vector<uint32_t>> by(nkey);
             for (uint32_t u = 0; u < nu; u++)
                 for (int i = 0; i < L.units[u].nitems; i++)
-                    by[2 * (size_t)level[u] + lane[u]].push_back(L.units[u].item_base + i);
+                    by[ukey(u)].push_back(L.units[u].item_base + i);
             for (size_t k = 0; k < by.size(); k++)
                 if (!by[k].empty()) {
                     node_of[k] = (int)lv.size();
-                    node_lane.push_back((int)(k & 1));
+                    node_lane.push_back((int)((k / NK) & 1));
+                    node_level.push_back((int)(k / NK / 2));
+                    e->level_gemm.push_back((char)(k % NK));
                     lv.push_back(std::move(by[k]));
                 }
         }
         const int nnode = (int)lv.size();
         // node of a unit; units without items (none are lowered) map to -1
-        auto unode = [&](uint32_t u) { return node_of[2 * (size_t)level[u] + lane[u]]; };
+        auto unode = [&](uint32_t u) { return node_of[ukey(u)]; };
         std::vector<std::vector<int>> node_deps(nnode);
-        int last[2] = {-1, -1};
-        for (int k = 0; k < nnode; k++) {
-            const int ln = node_lane[k];
-            if (last[ln] >= 0)
-                node_deps[k].push_back(last[ln]);
-            last[ln] = k;
+        {
+            // per lane: the nodes of the last level seen, and of the level before
+            std::vector<int> prev[2], cur[2];
+            int cur_level[2] = {-1, -1};
+            for (int k = 0; k < nnode; k++) {
+                const int ln = node_lane[k];
+                if (node_level[k] != cur_level[ln]) {
+                    prev[ln] = cur[ln];
+                    cur[ln].clear();
+                    cur_level[ln] = node_level[k];
+                }
+                node_deps[k] = prev[ln];
+                cur[ln].push_back(k);
+            }
         }
         for (uint32_t u = 0; u < nu; u++)
             for (uint32_t v : L.succ[u])
@@ -454,33 +504,6 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         for (uint32_t u = 0; u < nu; u++)
             if (L.units[u].nitems > 0)
                 e->level_flops[unode(u)] += unit_flops(L.units[u]);
-        // a node whose items are all GEMM/SYRK blocks runs on k_batch_gemm,
-        // one of only small diagonal factorizations / conversions on
-        // k_batch_small (both with small LDS, so several workgroups share a CU)
-        // a node of only TRSM items with inverted diagonal blocks runs on
-        // k_batch_trsm (8 waves per workgroup)
-        // ... and one of small panel solves (order <= 128) on k_batch_trsm_small
-        e->level_gemm.assign(nnode, LEVEL_GENERIC);
-        for (int l = 0; l < nnode; l++) {
-            bool g = true, sm = true, tr = true, ts = true, tm = true;
-            int tmax = 0;  // largest triangle of the level's TRSM / TRINV items
-            for (uint32_t it : lv[l]) {
-                const UnitDesc &d = L.units[item_unit[it]];
-                g = g && gemm_only_op(d.op);
-                sm = sm && small_only_op(d.op);
-                tr = tr && d.op == OP_TRSM && (d.flags & F_INV);
-                ts = ts && d.op == OP_TRSM && !(d.flags & F_INV) && d.m <= SMALL_MAX;
-                tm = tm && (d.op == OP_TRINV || (d.op == OP_TRSM && (d.flags & F_INV)));
-                tmax = std::max(tmax, (int)d.m);
-            }
-            tm = tm && tmax <= 512;
-            e->level_gemm[l] = g    ? LEVEL_GEMM
-                               : sm ? LEVEL_SMALL
-                               : tm ? (tmax <= 256 ? LEVEL_TRSM_256 : LEVEL_TRSM_512)
-                               : tr ? LEVEL_TRSM
-                               : ts ? LEVEL_TRSM_SMALL
-                                    : LEVEL_GENERIC;
-        }
         HIPCHK(hipGraphCreate(&e->graph, 0));
         std::vector<hipGraphNode_t> gnodes(nnode);
         for (int l = 0; l < nnode; l++) {
@@ -895,15 +918,38 @@ int colmax_abs(const double *M, int ld, int m, int n, double *out, hipStream_t s
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// ||A||_inf = max_i sum_j |A(i, j)|: one thread per row (a wave reads 64
-// consecutive rows of a column: coalesced), then an atomic max.
-__global__ void k_rowsum_max(const double *A, size_t lda, int m, int n, unsigned long long *out)
+// ||A||_inf = max_i sum_j |A(i, j)|.  Pass 1: thread (row i, column chunk c)
+// sums |A(i, j)| over the chunk's INF_KC columns into W[c m + i] (a wave
+// reads 64 consecutive rows of a column: coalesced; eight loads in flight).
+// Pass 2: per row, the chunks in order, then the maximum (an atomic max of
+// non-negative doubles, order-independent, on their bit patterns).
+constexpr int INF_KC = 256;
+
+__global__ void k_rowsum_partial(const double *A, size_t lda, int m, int n, double *W)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j0 = blockIdx.y * INF_KC, j1 = min(n, j0 + INF_KC);
+    if (i >= m)
+        return;
+    const double *a = A + i;
+    double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int j = j0;
+    for (; j + 8 <= j1; j += 8)
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            s[u] += fabs(a[(size_t)(j + u) * lda]);
+    for (; j < j1; j++)
+        s[0] += fabs(a[(size_t)j * lda]);
+    W[(size_t)blockIdx.y * m + i] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+__global__ void k_rowsum_max(const double *W, int m, int nchunk, unsigned long long *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double sum = 0.0;
     if (i < m)
-        for (int j = 0; j < n; j++)
-            sum += fabs(A[i + j * lda]);
+        for (int c = 0; c < nchunk; c++)
+            sum += W[(size_t)c * m + i];
     __shared__ double red[256];
     red[threadIdx.x] = sum;
     __syncthreads();
@@ -916,12 +962,18 @@ __global__ void k_rowsum_max(const double *A, size_t lda, int m, int n, unsigned
         atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));
 }
 
-int inf_norm(const double *A, int lda, int m, int n, double *out, hipStream_t s)
+size_t inf_norm_work_doubles(int m, int n) { return (size_t)((n + INF_KC - 1) / INF_KC) * m; }
+
+int inf_norm(const double *A, int lda, int m, int n, double *out, double *work, hipStream_t s)
 {
     unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, s, o, 1, 0ull);
-    hipLaunchKernelGGL(k_rowsum_max, dim3((m + 255) / 256), dim3(256), 0, s, A, (size_t)lda, m,
-                       n, o);
+    if (m <= 0 || n <= 0)
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    const int nchunk = (n + INF_KC - 1) / INF_KC;
+    hipLaunchKernelGGL(k_rowsum_partial, dim3((m + 255) / 256, nchunk), dim3(256), 0, s, A,
+                       (size_t)lda, m, n, work);
+    hipLaunchKernelGGL(k_rowsum_max, dim3((m + 255) / 256), dim3(256), 0, s, work, m, nchunk, o);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -939,6 +991,90 @@ int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipSt
 {
     hipLaunchKernelGGL(k_add_matrix, dim3(1024), dim3(256), 0, s, X, (size_t)ldx, D, (size_t)ldd,
                        m, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Residual R = B - A X for up to 16 right-hand sides, split over K so the
+// n x n read of A (the whole cost) runs on thousands of workgroups.  Pass 1:
+// workgroup (row block of 64, K chunk c) forms P_c = A[rows, chunk] X[chunk, :]
+// with v_mfma_f64_16x16x4f64, wave w on rows 16w..16w+15, and stores it to
+// the partial buffer P[c] (column-major, leading dimension m).  Pass 2 sums
+// the chunks in a fixed order, so the result does not depend on scheduling.
+constexpr int RES_KC = 2048;  // K chunk of one workgroup
+constexpr int RES_KU = 8;     // k-steps of 4 whose loads are issued together
+
+__global__ __launch_bounds__(256) void k_residual_partial(const double *A, size_t lda,
+                                                          const double *X, size_t ldx, int m,
+                                                          int n, int nrhs, double *P)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int r0 = blockIdx.x * 64 + wave * 16, c = blockIdx.y;
+    const int k0 = c * RES_KC, k1 = min(n, k0 + RES_KC);
+    // lane (li, lk) supplies A[r0 + li][k + lk] and X[k + lk][li]; rows and
+    // columns past the matrix read a clamped address and contribute zero
+    const int row = min(r0 + li, m - 1), col = min(li, nrhs - 1);
+    const bool rok = r0 + li < m, cok = li < nrhs;
+    const double *a = A + row;
+    const double *x = X + (size_t)col * ldx;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k = k0; k < k1; k += 4 * RES_KU) {
+        double fa[RES_KU], fb[RES_KU];
+#pragma unroll
+        for (int u = 0; u < RES_KU; u++) {
+            const int kk = k + 4 * u + lk;
+            const int kc = min(kk, n - 1);
+            const bool kok = kk < k1;
+            fa[u] = (rok && kok) ? a[(size_t)kc * lda] : 0.0;
+            fb[u] = (cok && kok) ? x[kc] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < RES_KU; u++)
+            acc = mfma_f64(fa[u], fb[u], acc);
+    }
+    // acc[e] = D[4e + lk][li]: row r0 + 4e + lk, column li
+    double *p = P + (size_t)c * m * 16;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int r = r0 + 4 * e + lk;
+        if (r < m && li < nrhs)
+            p[r + (size_t)li * m] = acc[e];
+    }
+}
+
+__global__ void k_residual_reduce(const double *B, size_t ldb, const double *P, int m, int nrhs,
+                                  int nchunk, double *R, size_t ldr)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)m * nrhs;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = k % m, j = k / m;
+        double sum = 0.0;
+        for (int c = 0; c < nchunk; c++)
+            sum += P[(size_t)c * m * 16 + i + j * m];
+        R[i + j * ldr] = B[i + j * ldb] - sum;
+    }
+}
+
+size_t residual_partial_doubles(int m, int n)
+{
+    return (size_t)((n + RES_KC - 1) / RES_KC) * m * 16;
+}
+
+int residual(const double *A, int lda, const double *X, int ldx, const double *B, int ldb,
+             double *R, int ldr, int m, int n, int nrhs, double *P, hipStream_t s)
+{
+    if (m <= 0 || nrhs <= 0)
+        return 0;
+    const int nchunk = (n + RES_KC - 1) / RES_KC;
+    for (int j0 = 0; j0 < nrhs; j0 += 16) {
+        const int w = std::min(16, nrhs - j0);
+        if (n > 0)
+            hipLaunchKernelGGL(k_residual_partial, dim3((m + 63) / 64, nchunk), dim3(256), 0, s,
+                               A, (size_t)lda, X + (size_t)j0 * ldx, (size_t)ldx, m, n, w, P);
+        hipLaunchKernelGGL(k_residual_reduce, dim3(1024), dim3(256), 0, s,
+                           B + (size_t)j0 * ldb, (size_t)ldb, P, m, w, n > 0 ? nchunk : 0,
+                           R + (size_t)j0 * ldr, (size_t)ldr);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -30,6 +30,7 @@ enum LevelKernel : char {
     LEVEL_TRSM_SMALL = 4,
     LEVEL_TRSM_256 = 5,
     LEVEL_TRSM_512 = 6,
+    LEVEL_KINDS = 7,
 };
 
 struct Exec {
@@ -86,10 +87,17 @@ int sum_abs(const double *A, int lda, int m, int n, double *acc, hipStream_t s);
 int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, hipStream_t s);
 // out[j] = max_i |M(i, j)| for the n columns of M (m x n)
 int colmax_abs(const double *M, int ld, int m, int n, double *out, hipStream_t s);
-// *out = ||A||_inf (max row sum of |A|)
-int inf_norm(const double *A, int lda, int m, int n, double *out, hipStream_t s);
+// *out = ||A||_inf (max row sum of |A|), through a work buffer of
+// inf_norm_work_doubles(m, n) doubles
+size_t inf_norm_work_doubles(int m, int n);
+int inf_norm(const double *A, int lda, int m, int n, double *out, double *work, hipStream_t s);
 // X += D
 int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipStream_t s);
+// R = B - A X (A m x n, X n x nrhs) through a K-split partial buffer P of
+// residual_partial_doubles(m, n) doubles; deterministic.
+size_t residual_partial_doubles(int m, int n);
+int residual(const double *A, int lda, const double *X, int ldx, const double *B, int ldb,
+             double *R, int ldr, int m, int n, int nrhs, double *P, hipStream_t s);
 // conditioned test matrices: A = diag(kappa^(-i/(n-1))); acc += x.y; A <- H A H
 int gen_cond_start(double *A, int n, int lda, double kappa, hipStream_t s);
 // t consecutive b x b identity tiles (tile k at D + k b^2, leading dimension b)

@@ -112,13 +112,15 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     // info cell and the norms
     const int b = n / t;
     const int ninv = opts->kind == MU_SOLVE_LU ? 2 : 1;
-    DevBuf dF, dR, dY, dD, dN, dI;
+    DevBuf dF, dR, dY, dD, dN, dI, dP;
     HK(hipMalloc(&dF.p, (size_t)n * n * sizeof(double)));
     HK(hipMalloc(&dR.p, (size_t)n * nrhs * sizeof(double)));
     HK(hipMalloc(&dY.p, (size_t)n * nrhs * sizeof(double)));
     HK(hipMalloc(&dD.p, (size_t)ninv * n * b * sizeof(double)));
     HK(hipMalloc(&dN.p, (size_t)(2 * nrhs + 1) * sizeof(double)));
     HK(hipMalloc(&dI.p, sizeof(int)));
+    HK(hipMalloc(&dP.p, std::max(mu::residual_partial_doubles(n, n),
+                                 mu::inf_norm_work_doubles(n, n)) * sizeof(double)));
     double *F = (double *)dF.p, *R = (double *)dR.p, *Y = (double *)dY.p;
     double *Linv = (double *)dD.p, *Uinv = Linv + (size_t)n * b;
     double *rn = (double *)dN.p, *xn = rn + nrhs, *an = xn + nrhs;
@@ -126,8 +128,8 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
 
 
     // Graphs over F, R, Y: the factorization (with the inversion of its
-    // diagonal tiles, D_k = T_kk^-1 by a TRSM against the identity, once per
-    // factorization) and the solve.  With the inverses every step of the
+    // diagonal tiles, D_k = T_kk^-1 by a TRSM against the identity, in the
+    // same graph) and the solve.  With the inverses every step of the
     // substitutions is a GEMM over all of its output blocks instead of a
     // single-workgroup panel solve:
     //   forward   Y_k = Linv_k R_k;          R_i -= L_ik Y_k      (i > k)
@@ -136,30 +138,33 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     // leaving the solution in R.
     auto fb = [&](int i, int j) { return F + (size_t)i * b + (size_t)j * b * n; };
     auto dinv = [&](double *D, int k) { return D + (size_t)k * b * b; };
-    auto build = [&](int prec, Dag &fd, Ex &fx, Dag &id, Ex &ix, Dag &sd, Ex &sx) -> int {
+    auto build = [&](int prec, Dag &fd, Ex &fx, Dag &sd, Ex &sx) -> int {
         CK(mu_dag_create(&fd.h));
         if (opts->kind == MU_SOLVE_LU)
             CK(mu_record_tiled_lu(fd.h, F, n, t, n, info, prec));
         else
             CK(mu_record_tiled_cholesky(fd.h, F, n, t, n, info, prec));
-        CK(mu_exec_create(fd.h, MU_MODE_TILED, nullptr, &fx.h));
-        // inverses of the diagonal tiles (independent tasks)
-        CK(mu_dag_create(&id.h));
-        void *key = id.h;
-        CK(mu_dag_capture_begin(id.h, key));
+        // the inverses of the diagonal tiles join the factorization's DAG:
+        // the task for tile k reads tile (k, k) only, so it runs as soon as
+        // that tile is factored, beside the rest of the factorization.  Its
+        // output is workspace tile (k, t) (L) or (k, t + 1) (U), outside the
+        // matrix's tile keys.
+        void *key = fd.h;
+        CK(mu_dag_capture_begin(fd.h, key));
         for (int k = 0; k < t; k++) {
-            CK(mu_dag_begin_task(id.h, k, 0, nullptr, 0, nullptr));
+            const int rd[2] = {k, k};
+            CK(mu_dag_begin_task(fd.h, k, t, rd, 1, nullptr));
             CK(mu_dtrsm(key, 'L', 'L', 'N', opts->kind == MU_SOLVE_LU ? 'U' : 'N', b, b, 1.0,
                         fb(k, k), n, dinv(Linv, k), b));
-            CK(mu_dag_end_task(id.h));
+            CK(mu_dag_end_task(fd.h));
             if (opts->kind == MU_SOLVE_LU) {
-                CK(mu_dag_begin_task(id.h, k, 1, nullptr, 0, nullptr));
+                CK(mu_dag_begin_task(fd.h, k, t + 1, rd, 1, nullptr));
                 CK(mu_dtrsm(key, 'L', 'U', 'N', 'N', b, b, 1.0, fb(k, k), n, dinv(Uinv, k), b));
-                CK(mu_dag_end_task(id.h));
+                CK(mu_dag_end_task(fd.h));
             }
         }
-        CK(mu_dag_capture_end(id.h, key));
-        CK(mu_exec_create(id.h, MU_MODE_TILED, nullptr, &ix.h));
+        CK(mu_dag_capture_end(fd.h, key));
+        CK(mu_exec_create(fd.h, MU_MODE_TILED, nullptr, &fx.h));
         // the solve: tile k of R is (k, 0), of Y is (k, 1)
         CK(mu_dag_create(&sd.h));
         key = sd.h;
@@ -198,12 +203,11 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
         CK(mu_exec_create(sd.h, MU_MODE_TILED, nullptr, &sx.h));
         return 0;
     };
-    auto factor = [&](Ex &fx, Ex &ix) -> int {
+    auto factor = [&](Ex &fx) -> int {
         KK(mu::copy_matrix(A, lda, F, n, n, n, s));
         CK(mu_info_reset(info, s));
-        CK(mu_exec_launch(fx.h, s));
         KK(mu::identity_tiles(Linv, b, t * ninv, s));
-        CK(mu_exec_launch(ix.h, s));
+        CK(mu_exec_launch(fx.h, s));
         HK(hipMemcpyAsync(&res->info, info, sizeof(int), hipMemcpyDeviceToHost, s));
         HK(hipStreamSynchronize(s));
         if (res->info == INT32_MAX)
@@ -213,8 +217,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     // r = b - A x; returns per-column ||r||_inf, ||x||_inf on the host
     std::vector<double> hr(nrhs), hx(nrhs), hb(nrhs);
     auto residual = [&]() -> int {
-        KK(mu::copy_matrix(B, ldb, R, n, n, nrhs, s));
-        CK(mu_dgemm(stream, 'N', 'N', n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, R, n));
+        KK(mu::residual(A, lda, X, ldx, B, ldb, R, n, n, n, nrhs, (double *)dP.p, s));
         KK(mu::colmax_abs(R, n, n, nrhs, rn, s));
         KK(mu::colmax_abs(X, ldx, n, nrhs, xn, s));
         HK(hipMemcpyAsync(hr.data(), rn, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -224,15 +227,15 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     };
 
     // graphs are built (lowered, uploaded, instantiated) before the clock starts
-    Dag fd, idg, sd;
-    Ex fx, ix, sx;
-    CK(build(opts->prec, fd, fx, idg, ix, sd, sx));
+    Dag fd, sd;
+    Ex fx, sx;
+    CK(build(opts->prec, fd, fx, sd, sx));
     Ev e0, e1, e2;
     HK(hipEventCreate(&e0.e));
     HK(hipEventCreate(&e1.e));
     HK(hipEventCreate(&e2.e));
     HK(hipEventRecord(e0.e, s));
-    KK(mu::inf_norm(A, lda, n, n, an, s));
+    KK(mu::inf_norm(A, lda, n, n, an, (double *)dP.p, s));
     KK(mu::colmax_abs(B, ldb, n, nrhs, rn, s));
     HK(hipMemcpyAsync(&res->anorm, an, sizeof(double), hipMemcpyDeviceToHost, s));
     HK(hipMemcpyAsync(hb.data(), rn, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -254,7 +257,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
         return true;
     };
 
-    CK(factor(fx, ix));
+    CK(factor(fx));
     HK(hipEventRecord(e1.e, s));
     if (res->info != 0) {
         mu_solve_ir_impl_set_error("mu_solve_ir: the factorization failed (info > 0)");
@@ -286,10 +289,10 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     res->iters = it;
     if (!res->converged && opts->fallback && opts->prec != MU_PREC_FP64) {
         // LAPACK's fallback: refactor in full precision and solve
-        Dag fd2, id2, sd2;
-        Ex fx2, ix2, sx2;
-        CK(build(MU_PREC_FP64, fd2, fx2, id2, ix2, sd2, sx2));
-        CK(factor(fx2, ix2));
+        Dag fd2, sd2;
+        Ex fx2, sx2;
+        CK(build(MU_PREC_FP64, fd2, fx2, sd2, sx2));
+        CK(factor(fx2));
         if (res->info != 0) {
             mu_solve_ir_impl_set_error("mu_solve_ir: the FP64 factorization failed (info > 0)");
             return MU_ERUN;
