@@ -287,6 +287,13 @@ struct Builder {
             UnitDesc d = base(op);
             d.gm = cdiv(op.m, GEMM_BM);
             d.gn = cdiv(op.n, GEMM_BN);
+            if (op.n <= 16 && !(op.flags & (F_TB | F_LOW_F32 | F_LOW_BF16))) {
+                d.flags |= F_NARROW;
+                int u = add_unit(d);
+                f.src = {u};
+                f.snk = {u};
+                return true;
+            }
             if (pair_aligned(op.a, op.flags) && pair_aligned(op.b, op.flags) && op.lda % 2 == 0 &&
                 op.ldb % 2 == 0 && op.m % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;

@@ -1068,6 +1068,140 @@ __device__ __forceinline__ void gemm_dispatch_vec(const UnitDesc &u, int bi, int
 }
 
 // GEMM or SYRK item `it` of unit `u` (the only kinds k_batch_gemm runs).
+// ---------------------------------------------------------------------------
+// GEMM block for narrow outputs (units flagged F_NARROW: n <= 16 columns, the
+// right-hand sides of a substitution step; FP64, op(B) = B).  An item is 64
+// output rows x all n columns.  Each of the 4 waves runs every 16-row tile of
+// the block over its own quarter of K (k-chunks of 16 taken round-robin),
+// reading the MFMA fragments straight from global memory -- lane (li, lk)
+// supplies op(A)[row li][k + lk] and B[k + lk][col li] -- with the next
+// chunk's loads issued before the current chunk's MFMAs; the four partial
+// sums are then added through LDS (waves 2, 3 into 0, 1, then 1 into 0).
+// The 64 x 64 block would spend 3/4 of its MFMAs on absent columns.
+// ---------------------------------------------------------------------------
+template <bool TA>
+__device__ __attribute__((always_inline)) inline void gemm_narrow(const UnitDesc &u, int bi,
+                                                                  double *smem)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int i0 = bi * 64, K = u.k, ncol = u.n;
+    const MU_G double *a = gp(u.a);
+    const MU_G double *b = gp(u.b);
+    const size_t lda = u.lda, ldb = u.ldb;
+    const bool cok = li < ncol;
+    const MU_G double *bc = b + (size_t)min(li, ncol - 1) * ldb;
+    const MU_G double *ar[4];
+    bool rok[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        const int r = i0 + 16 * x + li;
+        rok[x] = r < u.m;
+        const int rc = min(r, u.m - 1);
+        ar[x] = TA ? a + (size_t)rc * lda : a + rc;
+    }
+    v4d acc[4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+        acc[x] = v4d{0.0, 0.0, 0.0, 0.0};
+    constexpr int KU = 4;  // k-steps of 4 per chunk
+    double fa[2][KU][4], fb[2][KU];
+    auto load = [&](double (&ga)[KU][4], double (&gb)[KU], int k) {
+#pragma unroll
+        for (int q = 0; q < KU; q++) {
+            const int kk = k + 4 * q + lk;
+            const bool kok = kk < K;
+            const int kc = min(kk, K - 1);
+            gb[q] = (cok && kok) ? bc[kc] : 0.0;
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+                ga[q][x] = (rok[x] && kok) ? (TA ? ar[x][kc] : ar[x][(size_t)kc * lda]) : 0.0;
+        }
+    };
+    auto mma = [&](const double (&ga)[KU][4], const double (&gb)[KU]) {
+#pragma unroll
+        for (int q = 0; q < KU; q++)
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+                acc[x] = mfma_f64(ga[q][x], gb[q], acc[x]);
+    };
+    constexpr int CH = 4 * KU;  // k per chunk
+    int k = wave * CH;
+    if (K > 0 && k < K) {
+        load(fa[0], fb[0], k);
+        for (;;) {
+            const int kn = k + 4 * CH;
+            if (kn < K) {
+                load(fa[1], fb[1], kn);
+                mma(fa[0], fb[0]);
+            } else {
+                mma(fa[0], fb[0]);
+                break;
+            }
+            const int kn2 = kn + 4 * CH;
+            if (kn2 < K) {
+                load(fa[0], fb[0], kn2);
+                mma(fa[1], fb[1]);
+            } else {
+                mma(fa[1], fb[1]);
+                break;
+            }
+            k = kn2;
+        }
+    }
+    // reduce the four waves' partial sums: LDS slot [wave & 1][x][e][lane]
+    if (wave >= 2)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                smem[(((wave - 2) * 4 + x) * 4 + e) * 64 + lane] = acc[x][e];
+    lds_barrier();
+    if (wave < 2)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                acc[x][e] += smem[((wave * 4 + x) * 4 + e) * 64 + lane];
+    lds_barrier();
+    if (wave == 1)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                smem[(x * 4 + e) * 64 + lane] = acc[x][e];
+    lds_barrier();
+    if (wave != 0)
+        return;
+    const double alpha = u.alpha, beta = u.beta;
+    MU_G double *c = gp(u.c);
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            // acc[x][e] = D[4e + lk][li] of row tile x
+            const int r = i0 + 16 * x + 4 * e + lk;
+            const double v = acc[x][e] + smem[(x * 4 + e) * 64 + lane];
+            if (r < u.m && cok) {
+                MU_G double *cp = c + r + (size_t)li * u.ldc;
+                *cp = (beta != 0.0 ? beta * *cp : 0.0) + alpha * v;
+            }
+        }
+}
+static_assert(2 * 4 * 4 * 64 <= LDS_GEMM, "narrow GEMM reduction fits the GEMM LDS");
+
+// Narrow GEMM item (F_NARROW units only; kept out of run_gemm_item so the
+// 64 x 64 block kernels keep their register budget).
+__device__ __attribute__((always_inline)) inline void run_gemm_narrow_item(const UnitDesc &u,
+                                                                           int it, double *smem)
+{
+    if (u.flags & F_TA)
+        gemm_narrow<true>(u, it, smem);
+    else
+        gemm_narrow<false>(u, it, smem);
+}
+
+// GEMM/SYRK item of a unit without F_NARROW.
 __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDesc &u, int it,
                                                                     double *smem)
 {
@@ -1185,6 +1319,11 @@ __device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u
 {
     switch (u.op) {
     case OP_GEMM:
+        if (u.flags & F_NARROW)
+            run_gemm_narrow_item(u, it, smem);
+        else
+            run_gemm_item(u, it, smem);
+        break;
     case OP_SYRK: run_gemm_item(u, it, smem); break;
     case OP_TRSM:
         switch ((u.flags >> 3) & 0x67) {  // T_TRANS, Z_ROWS, UNIT, REV, INV

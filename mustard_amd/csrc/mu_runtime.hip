@@ -82,6 +82,17 @@ __global__ __launch_bounds__(WG_THREADS) void k_batch_small(const UnitDesc *unit
     run_small_item(d, (int)(item - d.item_base), smem);
 }
 
+// GEMMs with at most 16 output columns (substitution steps of a solve).
+__global__ __launch_bounds__(WG_THREADS) void k_batch_gemm_narrow(const UnitDesc *units,
+                                                                  const uint32_t *item_unit,
+                                                                  const uint32_t *items)
+{
+    __shared__ double smem[LDS_GEMM];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    run_gemm_narrow_item(d, (int)(item - d.item_base), smem);
+}
+
 // Panel solves of order <= SMALL_MAX (inside nested diagonal factorizations).
 __global__ __launch_bounds__(WG_THREADS) void k_batch_trsm_small(const UnitDesc *units,
                                                                  const uint32_t *item_unit,
@@ -132,6 +143,7 @@ static const void *level_kernel(int kind)
     case LEVEL_SMALL: return reinterpret_cast<const void *>(k_batch_small);
     case LEVEL_TRSM: return reinterpret_cast<const void *>(k_batch_trsm);
     case LEVEL_TRSM_SMALL: return reinterpret_cast<const void *>(k_batch_trsm_small);
+    case LEVEL_GEMM_NARROW: return reinterpret_cast<const void *>(k_batch_gemm_narrow);
     case LEVEL_TRSM_256: return reinterpret_cast<const void *>(k_batch_trsm_mid<256>);
     case LEVEL_TRSM_512: return reinterpret_cast<const void *>(k_batch_trsm_mid<512>);
     default: return reinterpret_cast<const void *>(k_batch);
@@ -294,7 +306,7 @@ int run_units(const Lowered &low, hipStream_t s, std::string &err)
     for (const UnitDesc &d : low.units) {
         if (d.nitems <= 0)
             continue;
-        if (gemm_only_op(d.op))
+        if (gemm_only_op(d.op) && !(d.flags & F_NARROW))
             hipLaunchKernelGGL(k_unit_gemm, dim3(d.nitems), dim3(WG_THREADS), 0, s, d);
         else
             hipLaunchKernelGGL(k_unit, dim3(d.nitems), dim3(WG_THREADS), 0, s, d);
@@ -402,6 +414,8 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         // 8-wave k_batch_trsm, small solves without inverses on
         // k_batch_trsm_small; anything else on the generic k_batch
         auto ukind = [&](const UnitDesc &d) -> int {
+            if (d.op == OP_GEMM && (d.flags & F_NARROW))
+                return LEVEL_GEMM_NARROW;
             if (gemm_only_op(d.op))
                 return LEVEL_GEMM;
             if (small_only_op(d.op))

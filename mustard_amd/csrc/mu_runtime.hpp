@@ -30,7 +30,8 @@ enum LevelKernel : char {
     LEVEL_TRSM_SMALL = 4,
     LEVEL_TRSM_256 = 5,
     LEVEL_TRSM_512 = 6,
-    LEVEL_KINDS = 7,
+    LEVEL_GEMM_NARROW = 7,
+    LEVEL_KINDS = 8,
 };
 
 struct Exec {

@@ -40,6 +40,12 @@ GEMM_CASES = [
     (1000, 1000, 1000, "N", "T", 24, -1.0, 1.0),   # Cholesky trailing update shape
     (125, 125, 125, "N", "N", 875, -1.0, 1.0),     # nested sub-tile, odd offsets
     (64, 16, 1000, "N", "N", 0, -1.0, 1.0),
+    # n <= 16 (substitution steps of a solve): the narrow-output path
+    (1000, 16, 1000, "T", "N", 0, -1.0, 1.0),
+    (1000, 16, 1000, "N", "N", 5, 1.0, 0.0),
+    (100, 3, 37, "N", "N", 1, 0.5, 2.0),
+    (65, 1, 1, "T", "N", 0, -1.0, 1.0),
+    (130, 15, 67, "T", "N", 2, 1.0, 0.0),
 ]
 
 
