@@ -289,6 +289,7 @@ struct Builder {
             d.gn = cdiv(op.n, GEMM_BN);
             if (op.n <= 16 && !(op.flags & (F_TB | F_LOW_F32 | F_LOW_BF16))) {
                 d.flags |= F_NARROW;
+                d.gm = cdiv(op.m, 16);  // 16-row items (NARROW_ROWS)
                 int u = add_unit(d);
                 f.src = {u};
                 f.snk = {u};

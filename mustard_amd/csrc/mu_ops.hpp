@@ -1070,60 +1070,50 @@ __device__ __forceinline__ void gemm_dispatch_vec(const UnitDesc &u, int bi, int
 // GEMM or SYRK item `it` of unit `u` (the only kinds k_batch_gemm runs).
 // ---------------------------------------------------------------------------
 // GEMM block for narrow outputs (units flagged F_NARROW: n <= 16 columns, the
-// right-hand sides of a substitution step; FP64, op(B) = B).  An item is 64
-// output rows x all n columns.  Each of the 4 waves runs every 16-row tile of
-// the block over its own quarter of K (k-chunks of 16 taken round-robin),
-// reading the MFMA fragments straight from global memory -- lane (li, lk)
-// supplies op(A)[row li][k + lk] and B[k + lk][col li] -- with the next
-// chunk's loads issued before the current chunk's MFMAs; the four partial
-// sums are then added through LDS (waves 2, 3 into 0, 1, then 1 into 0).
-// The 64 x 64 block would spend 3/4 of its MFMAs on absent columns.
+// right-hand sides of a substitution step; FP64, op(B) = B).  An item is one
+// 16-row tile x all n columns, so a level of these (one 1000-row step is 63
+// items) spreads over many CUs.  The 4 waves split K (k-chunks of 32 taken
+// round-robin) and read the MFMA fragments straight from global memory --
+// lane (li, lk) supplies op(A)[row li][k + lk] and B[k + lk][col li] -- with
+// the next chunk's loads issued before the current chunk's MFMAs; the four
+// partial tiles are added through LDS.  A 64 x 64 block would spend 3/4 of
+// its MFMAs on absent columns and leave most CUs idle.
 // ---------------------------------------------------------------------------
+constexpr int NARROW_ROWS = 16;  // output rows per narrow item
+
 template <bool TA>
 __device__ __attribute__((always_inline)) inline void gemm_narrow(const UnitDesc &u, int bi,
                                                                   double *smem)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int i0 = bi * 64, K = u.k, ncol = u.n;
+    const int i0 = bi * NARROW_ROWS, K = u.k, ncol = u.n;
     const MU_G double *a = gp(u.a);
     const MU_G double *b = gp(u.b);
     const size_t lda = u.lda, ldb = u.ldb;
     const bool cok = li < ncol;
     const MU_G double *bc = b + (size_t)min(li, ncol - 1) * ldb;
-    const MU_G double *ar[4];
-    bool rok[4];
-#pragma unroll
-    for (int x = 0; x < 4; x++) {
-        const int r = i0 + 16 * x + li;
-        rok[x] = r < u.m;
-        const int rc = min(r, u.m - 1);
-        ar[x] = TA ? a + (size_t)rc * lda : a + rc;
-    }
-    v4d acc[4];
-#pragma unroll
-    for (int x = 0; x < 4; x++)
-        acc[x] = v4d{0.0, 0.0, 0.0, 0.0};
-    constexpr int KU = 4;  // k-steps of 4 per chunk
-    double fa[2][KU][4], fb[2][KU];
-    auto load = [&](double (&ga)[KU][4], double (&gb)[KU], int k) {
+    const int r = i0 + li;
+    const bool rok = r < u.m;
+    const int rc = min(r, u.m - 1);
+    const MU_G double *ar = TA ? a + (size_t)rc * lda : a + rc;
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    constexpr int KU = 8;  // k-steps of 4 per chunk
+    double fa[2][KU], fb[2][KU];
+    auto load = [&](double (&ga)[KU], double (&gb)[KU], int k) {
 #pragma unroll
         for (int q = 0; q < KU; q++) {
             const int kk = k + 4 * q + lk;
             const bool kok = kk < K;
             const int kc = min(kk, K - 1);
             gb[q] = (cok && kok) ? bc[kc] : 0.0;
-#pragma unroll
-            for (int x = 0; x < 4; x++)
-                ga[q][x] = (rok[x] && kok) ? (TA ? ar[x][kc] : ar[x][(size_t)kc * lda]) : 0.0;
+            ga[q] = (rok && kok) ? (TA ? ar[kc] : ar[(size_t)kc * lda]) : 0.0;
         }
     };
-    auto mma = [&](const double (&ga)[KU][4], const double (&gb)[KU]) {
+    auto mma = [&](const double (&ga)[KU], const double (&gb)[KU]) {
 #pragma unroll
         for (int q = 0; q < KU; q++)
-#pragma unroll
-            for (int x = 0; x < 4; x++)
-                acc[x] = mfma_f64(ga[q][x], gb[q], acc[x]);
+            acc = mfma_f64(ga[q], gb[q], acc);
     };
     constexpr int CH = 4 * KU;  // k per chunk
     int k = wave * CH;
@@ -1149,46 +1139,29 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow(const UnitDesc
             k = kn2;
         }
     }
-    // reduce the four waves' partial sums: LDS slot [wave & 1][x][e][lane]
-    if (wave >= 2)
+    // waves 1..3 hand their partial tiles to wave 0: LDS slot [wave - 1][e][lane]
+    if (wave > 0)
 #pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                smem[(((wave - 2) * 4 + x) * 4 + e) * 64 + lane] = acc[x][e];
-    lds_barrier();
-    if (wave < 2)
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                acc[x][e] += smem[((wave * 4 + x) * 4 + e) * 64 + lane];
-    lds_barrier();
-    if (wave == 1)
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                smem[(x * 4 + e) * 64 + lane] = acc[x][e];
+        for (int e = 0; e < 4; e++)
+            smem[((wave - 1) * 4 + e) * 64 + lane] = acc[e];
     lds_barrier();
     if (wave != 0)
         return;
     const double alpha = u.alpha, beta = u.beta;
     MU_G double *c = gp(u.c);
 #pragma unroll
-    for (int x = 0; x < 4; x++)
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            // acc[x][e] = D[4e + lk][li] of row tile x
-            const int r = i0 + 16 * x + 4 * e + lk;
-            const double v = acc[x][e] + smem[(x * 4 + e) * 64 + lane];
-            if (r < u.m && cok) {
-                MU_G double *cp = c + r + (size_t)li * u.ldc;
-                *cp = (beta != 0.0 ? beta * *cp : 0.0) + alpha * v;
-            }
+    for (int e = 0; e < 4; e++) {
+        // acc[e] = D[4e + lk][li]: row i0 + 4e + lk, column li
+        const int row = i0 + 4 * e + lk;
+        const double v = acc[e] + (smem[e * 64 + lane] + smem[(4 + e) * 64 + lane] +
+                                   smem[(8 + e) * 64 + lane]);
+        if (row < u.m && cok) {
+            MU_G double *cp = c + row + (size_t)li * u.ldc;
+            *cp = (beta != 0.0 ? beta * *cp : 0.0) + alpha * v;
         }
+    }
 }
-static_assert(2 * 4 * 4 * 64 <= LDS_GEMM, "narrow GEMM reduction fits the GEMM LDS");
+static_assert(3 * 4 * 64 <= LDS_GEMM, "narrow GEMM reduction fits the GEMM LDS");
 
 // Narrow GEMM item (F_NARROW units only; kept out of run_gemm_item so the
 // 64 x 64 block kernels keep their register budget).

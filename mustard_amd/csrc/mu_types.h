@@ -38,7 +38,7 @@ enum Flag : int32_t {
     F_REV = 1 << 8,      // TRSM: row order of T and Z reversed (upper-triangular solves)
     F_INV = 1 << 9,      // TRSM: diagonal blocks solved with the inverses a TRINV unit left in b
     F_SRC_LOW = 1 << 10, // GEMM/SYRK with F_LOW_*: a and b are stored in that precision (not FP64)
-    F_NARROW = 1 << 11,  // GEMM (FP64, op(B) = B) with n <= 16: items are 64-row blocks, every
+    F_NARROW = 1 << 11,  // GEMM (FP64, op(B) = B) with n <= 16: items are 16-row tiles, every
                          // column in one item
 };
 
