@@ -758,6 +758,42 @@ int issue_gemm_low(void *key, bool tb, const Shadow &sh, int ai, int aj, int bi,
     return issue(key, op);
 }
 
+// Reduced-precision update reading the FP64 tiles themselves (rounded while
+// staged): the updates that feed the next panel use it, so they need not
+// wait for the shadow conversions.
+int issue_gemm_f64src(void *key, bool tb, const double *A, const double *B, int ld, double *C,
+                      int b, int prec)
+{
+    mu::OpRec op;
+    op.op = mu::OP_GEMM;
+    op.flags = (tb ? mu::F_TB : 0) | prec_flags(prec);
+    op.m = op.n = op.k = b;
+    op.alpha = -1.0;
+    op.beta = 1.0;
+    op.a = A;
+    op.lda = ld;
+    op.b = B;
+    op.ldb = ld;
+    op.c = C;
+    op.ldc = ld;
+    return issue(key, op);
+}
+
+int issue_syrk_f64src(void *key, const double *A, int ld, double *C, int b, int prec)
+{
+    mu::OpRec op;
+    op.op = mu::OP_SYRK;
+    op.flags = prec_flags(prec);
+    op.m = op.n = op.k = b;
+    op.alpha = -1.0;
+    op.beta = 1.0;
+    op.a = A;
+    op.lda = ld;
+    op.c = C;
+    op.ldc = ld;
+    return issue(key, op);
+}
+
 int issue_syrk_low(void *key, const Shadow &sh, int ai, int aj, double *C, int ldc, int prec)
 {
     mu::OpRec op;
@@ -789,6 +825,17 @@ int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info
         return rc;
     auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
     auto rd = [](std::initializer_list<int> v) { return std::vector<int>(v); };
+    // the shadow of tile (i, j) is written by a task of its own, tile key
+    // (i + t, j), outside the matrix's keys
+    auto record_cvt = [&](int i, int j) {
+        const int r1[2] = {i, j};
+        int c = mu_dag_begin_task(dag, i + t, j, r1, 1, nullptr);
+        if (!c)
+            c = issue_cvt(key, blk(i, j), lda, sh, i, j, prec);
+        if (!c)
+            c = mu_dag_end_task(dag);
+        return c;
+    };
     // lu_mustard.cu:343-535
     for (int k = 0; k < t && rc == 0; k++) {
         auto r0 = rd({k, k});
@@ -806,24 +853,30 @@ int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info
             rc = mu_dag_begin_task(dag, k, i, r.data(), 2, nullptr);
             if (!rc)
                 rc = mu_dtrsm(key, 'L', 'L', 'N', 'U', b, b, 1.0, blk(k, k), lda, blk(k, i), lda);
-            if (!rc && low)
-                rc = issue_cvt(key, blk(k, i), lda, sh, k, i, prec);
             if (!rc)
                 rc = mu_dag_end_task(dag);
+            if (!rc && low)
+                rc = record_cvt(k, i);
         }
         for (int i = k + 1; i < t && rc == 0; i++) {
             auto r = rd({k, k, i, k});
             rc = mu_dag_begin_task(dag, i, k, r.data(), 2, nullptr);
             if (!rc)
                 rc = mu_dtrsm(key, 'R', 'U', 'N', 'N', b, b, 1.0, blk(k, k), lda, blk(i, k), lda);
-            if (!rc && low)
-                rc = issue_cvt(key, blk(i, k), lda, sh, i, k, prec);
             if (!rc)
                 rc = mu_dag_end_task(dag);
+            if (!rc && low)
+                rc = record_cvt(i, k);
             for (int j = k + 1; j < t && rc == 0; j++) {
-                auto r3 = rd({i, k, k, j, i, j});
+                // with shadows, the updates of row and column k + 1 (the next
+                // panel's inputs) read the FP64 tiles; the others the shadows
+                const bool next = i == k + 1 || j == k + 1;
+                auto r3 = low && !next ? rd({i, j, i + t, k, k + t, j}) : rd({i, k, k, j, i, j});
                 rc = mu_dag_begin_task(dag, i, j, r3.data(), 3, nullptr);
-                if (!rc && low)
+                if (!rc && low && next)
+                    rc = issue_gemm_f64src(key, false, blk(i, k), blk(k, j), lda, blk(i, j), b,
+                                           prec);
+                else if (!rc && low)
                     rc = issue_gemm_low(key, false, sh, i, k, k, j, blk(i, j), lda, prec);
                 else if (!rc)
                     rc = mu_dgemm(key, 'N', 'N', b, b, b, -1.0, blk(i, k), lda, blk(k, j), lda,
@@ -858,6 +911,15 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
         return rc;
     auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
     auto rd = [](std::initializer_list<int> v) { return std::vector<int>(v); };
+    auto record_cvt = [&](int i, int j) {
+        const int r1[2] = {i, j};
+        int c = mu_dag_begin_task(dag, i + t, j, r1, 1, nullptr);
+        if (!c)
+            c = issue_cvt(key, blk(i, j), lda, sh, i, j, prec);
+        if (!c)
+            c = mu_dag_end_task(dag);
+        return c;
+    };
     // cholesky_mustard.cu:221-400
     for (int k = 0; k < t && rc == 0; k++) {
         auto r0 = rd({k, k});
@@ -874,24 +936,32 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
             rc = mu_dag_begin_task(dag, i, k, r.data(), 2, nullptr);
             if (!rc)
                 rc = mu_dtrsm(key, 'R', 'L', 'T', 'N', b, b, 1.0, blk(k, k), lda, blk(i, k), lda);
-            if (!rc && low)
-                rc = issue_cvt(key, blk(i, k), lda, sh, i, k, prec);
             if (!rc)
                 rc = mu_dag_end_task(dag);
+            if (!rc && low)
+                rc = record_cvt(i, k);
         }
         for (int i = k + 1; i < t && rc == 0; i++) {
-            auto r = rd({i, i, i, k});
+            // with shadows, the updates of column k + 1 (the next panel's
+            // inputs) read the FP64 tiles; the others the shadows
+            const bool next = i == k + 1;
+            auto r = low && !next ? rd({i, i, i + t, k}) : rd({i, i, i, k});
             rc = mu_dag_begin_task(dag, i, i, r.data(), 2, nullptr);
-            if (!rc && low)
+            if (!rc && low && next)
+                rc = issue_syrk_f64src(key, blk(i, k), lda, blk(i, i), b, prec);
+            else if (!rc && low)
                 rc = issue_syrk_low(key, sh, i, k, blk(i, i), lda, prec);
             else if (!rc)
                 rc = mu_dsyrk(key, 'L', 'N', b, b, -1.0, blk(i, k), lda, 1.0, blk(i, i), lda);
             if (!rc)
                 rc = mu_dag_end_task(dag);
             for (int j = i + 1; j < t && rc == 0; j++) {
-                auto r3 = rd({j, i, j, k, i, k});
+                auto r3 = low && !next ? rd({j, i, j + t, k, i + t, k}) : rd({j, i, j, k, i, k});
                 rc = mu_dag_begin_task(dag, j, i, r3.data(), 3, nullptr);
-                if (!rc && low)
+                if (!rc && low && next)
+                    rc = issue_gemm_f64src(key, true, blk(j, k), blk(i, k), lda, blk(j, i), b,
+                                           prec);
+                else if (!rc && low)
                     rc = issue_gemm_low(key, true, sh, j, k, i, k, blk(j, i), lda, prec);
                 else if (!rc)
                     rc = mu_dgemm(key, 'N', 'T', b, b, b, -1.0, blk(j, k), lda, blk(i, k), lda,

@@ -372,14 +372,15 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                 nlev = std::max(nlev, level[v] + 1);
             }
         // a unit belongs to a trailing update when its recorded task is a
-        // GEMM/SYRK task (not a piece of a nested factorization or solve);
+        // GEMM/SYRK task (not a piece of a nested factorization or solve) or
+        // the conversion of a panel tile to its reduced-precision shadow;
         // such a unit whose successors are all update units is bulk work ...
         auto update_task = [&](uint32_t u) {
             const int tk = L.units[u].task;
             if (tk < 0 || tk >= (int)dag.tasks.size() || dag.tasks[tk].ops.empty())
                 return false;
             for (const OpRec &op : dag.tasks[tk].ops)
-                if (!gemm_only_op(op.op))
+                if (!gemm_only_op(op.op) && op.op != OP_CVT)
                     return false;
             return true;
         };
