@@ -1011,49 +1011,63 @@ int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipSt
 
 // Residual R = B - A X for up to 16 right-hand sides, split over K so the
 // n x n read of A (the whole cost) runs on thousands of workgroups.  Pass 1:
-// workgroup (row block of 64, K chunk c) forms P_c = A[rows, chunk] X[chunk, :]
-// with v_mfma_f64_16x16x4f64, wave w on rows 16w..16w+15, and stores it to
-// the partial buffer P[c] (column-major, leading dimension m).  Pass 2 sums
-// the chunks in a fixed order, so the result does not depend on scheduling.
-constexpr int RES_KC = 2048;  // K chunk of one workgroup
-constexpr int RES_KU = 8;     // k-steps of 4 whose loads are issued together
+// workgroup (256 rows, K chunk c) forms P_c = A[rows, chunk] X[chunk, :]:
+// thread r reads row r of A down the chunk (a wave reads 64 consecutive rows
+// of a column: 512 contiguous bytes) and keeps 16 sums, one per right-hand
+// side, against X rows staged in LDS 128 at a time (read as broadcasts).
+// The stores go to the partial buffer P[c] (column-major, leading dimension
+// m).  Pass 2 sums the chunks in a fixed order, so the result does not
+// depend on scheduling.
+constexpr int RES_KC = 1024;  // K chunk of one workgroup
+constexpr int RES_KS = 128;   // X rows staged in LDS at a time
 
 __global__ __launch_bounds__(256) void k_residual_partial(const double *A, size_t lda,
                                                           const double *X, size_t ldx, int m,
                                                           int n, int nrhs, double *P)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int r0 = blockIdx.x * 64 + wave * 16, c = blockIdx.y;
+    __shared__ double xs[RES_KS * 16];
+    const int tid = threadIdx.x, c = blockIdx.y;
+    const int r = blockIdx.x * 256 + tid;
+    const bool rok = r < m;
+    const double *a = A + min(r, m - 1);
     const int k0 = c * RES_KC, k1 = min(n, k0 + RES_KC);
-    // lane (li, lk) supplies A[r0 + li][k + lk] and X[k + lk][li]; rows and
-    // columns past the matrix read a clamped address and contribute zero
-    const int row = min(r0 + li, m - 1), col = min(li, nrhs - 1);
-    const bool rok = r0 + li < m, cok = li < nrhs;
-    const double *a = A + row;
-    const double *x = X + (size_t)col * ldx;
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k = k0; k < k1; k += 4 * RES_KU) {
-        double fa[RES_KU], fb[RES_KU];
+    double acc[16];
 #pragma unroll
-        for (int u = 0; u < RES_KU; u++) {
-            const int kk = k + 4 * u + lk;
-            const int kc = min(kk, n - 1);
-            const bool kok = kk < k1;
-            fa[u] = (rok && kok) ? a[(size_t)kc * lda] : 0.0;
-            fb[u] = (cok && kok) ? x[kc] : 0.0;
+    for (int j = 0; j < 16; j++)
+        acc[j] = 0.0;
+    for (int ks = k0; ks < k1; ks += RES_KS) {
+        // stage X[ks .. ks + RES_KS)[0 .. 16) (zero past the chunk / nrhs)
+#pragma unroll
+        for (int q = 0; q < RES_KS * 16 / 256; q++) {
+            const int idx = tid + 256 * q, kq = idx >> 4, j = idx & 15;
+            const int kk = ks + kq;
+            xs[idx] = (kk < k1 && j < nrhs) ? X[(size_t)kk + (size_t)j * ldx] : 0.0;
         }
+        lds_barrier();
+        const int kn = min(RES_KS, k1 - ks);
+        for (int kq = 0; kq < kn; kq += 8) {
+            double av[8];
 #pragma unroll
-        for (int u = 0; u < RES_KU; u++)
-            acc = mfma_f64(fa[u], fb[u], acc);
+            for (int u = 0; u < 8; u++) {
+                const int kk = ks + kq + u;
+                av[u] = (rok && kq + u < kn) ? a[(size_t)min(kk, n - 1) * lda] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double *xr = xs + (kq + u) * 16;
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    acc[j] = fma(av[u], xr[j], acc[j]);
+            }
+        }
+        lds_barrier();
     }
-    // acc[e] = D[4e + lk][li]: row r0 + 4e + lk, column li
-    double *p = P + (size_t)c * m * 16;
+    if (rok) {
+        double *p = P + (size_t)c * m * 16 + r;
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const int r = r0 + 4 * e + lk;
-        if (r < m && li < nrhs)
-            p[r + (size_t)li * m] = acc[e];
+        for (int j = 0; j < 16; j++)
+            if (j < nrhs)
+                p[(size_t)j * m] = acc[j];
     }
 }
 
@@ -1084,8 +1098,8 @@ int residual(const double *A, int lda, const double *X, int ldx, const double *B
     for (int j0 = 0; j0 < nrhs; j0 += 16) {
         const int w = std::min(16, nrhs - j0);
         if (n > 0)
-            hipLaunchKernelGGL(k_residual_partial, dim3((m + 63) / 64, nchunk), dim3(256), 0, s,
-                               A, (size_t)lda, X + (size_t)j0 * ldx, (size_t)ldx, m, n, w, P);
+            hipLaunchKernelGGL(k_residual_partial, dim3((m + 255) / 256, nchunk), dim3(256), 0,
+                               s, A, (size_t)lda, X + (size_t)j0 * ldx, (size_t)ldx, m, n, w, P);
         hipLaunchKernelGGL(k_residual_reduce, dim3(1024), dim3(256), 0, s,
                            B + (size_t)j0 * ldb, (size_t)ldb, P, m, w, n > 0 ? nchunk : 0,
                            R + (size_t)j0 * ldr, (size_t)ldr);
