@@ -30,8 +30,8 @@ extern "C" {
 
 /* Execution modes of a recorded DAG. */
 #define MU_MODE_TILED 0    /* one hipGraph of batched kernels, one per DAG level (reference --tiled) */
-#define MU_MODE_SUBGRAPH 1 /* one persistent scheduler kernel (reference --subgraph); switched
-                              off in this build: mu_exec_create returns MU_ENOTSUP */
+#define MU_MODE_SUBGRAPH 1 /* one persistent scheduler kernel pulling ready items from HBM
+                              queues (reference --subgraph) */
 #define MU_MODE_STREAM 2   /* units launched one by one on the stream (debugging) */
 
 /* Precision of trailing GEMM/SYRK updates. */
@@ -44,7 +44,8 @@ typedef struct mu_exec_s *mu_exec_t;
 
 typedef struct {
     int grid;           /* persistent grid size, 0 = one workgroup per CU slot */
-    double timeout_s;   /* persistent: abort after this long without progress (0 = 30 s) */
+    double timeout_s;   /* persistent: abort after this long without progress (0 = 10 s);
+                           a host watchdog aborts a run after 3 * timeout_s + 1 s */
     int trace;          /* persistent: record per-item wall-clock stamps */
     int small_max;      /* diagonal factorizations above this order nest (0 = 128) */
     int trsm_max;       /* triangular solves above this order nest (0 = 256; at most 1024) */
@@ -52,6 +53,12 @@ typedef struct {
                            kernels, the trailing-update GEMM/SYRK tasks that feed only
                            other updates beside everything else (lookahead);
                            -1 = one chain */
+    int sm;             /* persistent: CU budget of each recorded task in percent of the
+                           GPU (the reference's --sm / FLAGS_OCCUP, mustard.h:82-85,
+                           lu_mustard.cu:290, 352-377): at most sm% of the workgroups run
+                           one task's items at once, and a task starts only while the
+                           budgets of the running tasks sum to less than 100.
+                           0 = no budget. */
 } mu_exec_opts_t;
 
 typedef struct {
@@ -321,6 +328,22 @@ int mu_residual_norms(const double *A, const double *F, int n, int lda, int chol
  * model of its device queue (mustard/broker_queue.h). */
 int mu_dag_emulate(mu_dag_t dag, int threads, double timeout_s, int bump_unit,
                    long long *results);
+/* The same with every option: `workgroups` emulated workgroups of four
+ * threads (waves) each, running the kernel's own worker loop with a barrier
+ * per workgroup; a host watchdog deadline; a stall of every producer between
+ * reserving ring positions and storing them; and the CU budget `sm` (percent,
+ * as in mu_exec_opts_t).  results[0..8] = the six above, then the most
+ * workgroups that ran one task's items at once, the per-task cap, and the
+ * most tasks that had items running at once. */
+typedef struct {
+    int workgroups;
+    double timeout_s;       /* device watchdog: no progress for this long */
+    double host_deadline_s; /* host watchdog (0 = none) */
+    int bump_unit;          /* >= 0: a never-satisfied dependency on this unit */
+    int producer_delay_us;
+    int sm;
+} mu_emu_opts_t;
+int mu_dag_emulate_ex(mu_dag_t dag, const mu_emu_opts_t *opts, long long *results);
 
 #ifdef __cplusplus
 }

@@ -174,13 +174,16 @@ class Dag:
 class Exec:
     """A lowered, uploaded DAG ready to launch."""
 
-    def __init__(self, dag: Dag, mode=MODE_TILED, grid=0, timeout_s=30.0, trace=False,
-                 small_max=0, trsm_max=0, lookahead=True):
+    def __init__(self, dag: Dag, mode=MODE_TILED, grid=0, timeout_s=10.0, trace=False,
+                 small_max=0, trsm_max=0, lookahead=True, sm=0):
         if isinstance(mode, str):
             mode = MODES[mode]
         self.mode = mode
         self.h = ctypes.c_void_p()
-        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, 1 if lookahead else -1)
+        # the executable points into buffers the DAG recorded; keep it alive
+        self._dag = dag
+        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, 1 if lookahead else -1,
+                        sm)
         call("mu_exec_create", dag.h, mode, ctypes.byref(opts), ctypes.byref(self.h))
 
     def launch(self, stream=None):

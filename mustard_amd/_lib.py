@@ -26,7 +26,13 @@ class MustardError(RuntimeError):
 class ExecOpts(ctypes.Structure):
     _fields_ = [("grid", ctypes.c_int), ("timeout_s", ctypes.c_double),
                 ("trace", ctypes.c_int), ("small_max", ctypes.c_int),
-                ("trsm_max", ctypes.c_int), ("lookahead", ctypes.c_int)]
+                ("trsm_max", ctypes.c_int), ("lookahead", ctypes.c_int), ("sm", ctypes.c_int)]
+
+
+class EmuOpts(ctypes.Structure):
+    _fields_ = [("workgroups", ctypes.c_int), ("timeout_s", ctypes.c_double),
+                ("host_deadline_s", ctypes.c_double), ("bump_unit", ctypes.c_int),
+                ("producer_delay_us", ctypes.c_int), ("sm", ctypes.c_int)]
 
 
 class ExecStats(ctypes.Structure):
@@ -123,6 +129,7 @@ SIGNATURES = {
     "mu_residual_norms": (_i, [_vp, _vp, _i, _i, _i, _vp, ctypes.POINTER(_d), ctypes.POINTER(_d),
                                ctypes.POINTER(_d), _vp]),
     "mu_dag_emulate": (_i, [_vp, _i, _d, _i, ctypes.POINTER(ctypes.c_longlong)]),
+    "mu_dag_emulate_ex": (_i, [_vp, ctypes.POINTER(EmuOpts), ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 _LIB = None

@@ -82,6 +82,10 @@ static void tiledCholesky(const cli::Args &cmdl, bool subgraph)
     cmdl.get({"grid"}, opts.grid);
     const std::string *trace = cmdl.param({"trace"});
     opts.trace = trace != nullptr;
+    // --sm: the CU budget of each task in the persistent scheduler
+    // (FLAGS_OCCUP, mustard.h:82-85); the tiled graph's level kernels mix the
+    // items of many tasks and run without one
+    opts.sm = subgraph ? cfg.smLimit : 0;
     mustard::Executable ex = creator.instantiate(
         stream_mode ? mustard::Mode::Stream
                     : (subgraph ? mustard::Mode::Subgraph : mustard::Mode::Tiled),

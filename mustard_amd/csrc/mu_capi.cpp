@@ -16,19 +16,8 @@
 #include "../../include/mustard_hip.h"
 #include "mu_dag.hpp"
 #include "mu_runtime.hpp"
+#include "mu_sched_host.hpp"
 
-namespace mu {
-struct EmuResult {
-    uint64_t executed = 0;
-    uint64_t duplicates = 0;
-    uint64_t order_violations = 0;
-    uint32_t err = 0;
-    uint32_t done = 0;
-    uint32_t nitems = 0;
-};
-bool emulate(const Dag &dag, const LowerOpts &lo, int threads, double timeout_s, int bump_unit,
-             EmuResult &res, std::string &err);
-}  // namespace mu
 
 struct mu_dag_s {
     mu::Dag dag;
@@ -300,8 +289,22 @@ int mu_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, 
         return fail(MU_EINVAL, "mu_dtrsm: bad side/uplo/trans/diag");
     if (m < 0 || n < 0 || ldb < std::max(1, m) || lda < std::max(1, side == 'L' ? m : n))
         return fail(MU_EINVAL, "mu_dtrsm: bad size or leading dimension");
-    if (alpha == 0.0)
-        return fail(MU_ENOTSUP, "mu_dtrsm: alpha = 0");
+    if (alpha == 0.0) {
+        // BLAS: alpha = 0 sets B to zero and never reads A; recorded as a
+        // GEMM with k = 0 and beta = 0 (C = 0 without reading C)
+        mu::OpRec z;
+        z.op = mu::OP_GEMM;
+        z.flags = mu::F_TB;  // the 64 x 64 block path, which skips every operand at k = 0
+        z.m = m;
+        z.n = n;
+        z.k = 0;
+        z.alpha = 0.0;
+        z.beta = 0.0;
+        z.lda = z.ldb = 1;
+        z.c = B;
+        z.ldc = ldb;
+        return issue(stream, z);
+    }
     // Reduce to T Y = alpha Z with T lower triangular:
     //   left:  op(A) X = B      -> T = op(A),    Z = B
     //   right: X op(A) = B      -> T = op(A)^T,  Z = B^T
@@ -502,11 +505,6 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
         return fail(MU_EINVAL, "mu_exec_create: bad arguments");
     if (dag->dag.current() >= 0)
         return fail(MU_ESTATE, "mu_exec_create: a task is still being recorded");
-    if (mode == MU_MODE_SUBGRAPH)
-        return fail(MU_ENOTSUP,
-                    "the persistent device scheduler (--subgraph) is switched off in this "
-                    "build: its first runs on the MI355X did not terminate, and the cause is "
-                    "not yet found; use --tiled (level-batched hipGraph)");
     mu::LowerOpts lo;
     mu::ExecOpts eo;
     if (opts) {
@@ -519,7 +517,13 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
         eo.grid = opts->grid;
         eo.trace = opts->trace;
         eo.lookahead = opts->lookahead >= 0 ? 1 : 0;
+        if (opts->sm < 0 || opts->sm > 100)
+            return fail(MU_EINVAL, "mu_exec_create: sm is a percentage of the GPU (0 = none)");
+        eo.sm = opts->sm;
     }
+    // the persistent kernel runs panel solves of order <= 256 (larger ones nest)
+    if (mode == MU_MODE_SUBGRAPH)
+        lo.trsm_max = std::min(lo.trsm_max, mu::PERSIST_TRSM_MAX);
     for (const auto &t : dag->dag.tasks)
         for (const auto &op : t.ops)
             if (op.flags & (mu::F_LOW_F32 | mu::F_LOW_BF16))
@@ -678,22 +682,49 @@ int mu_exec_destroy(mu_exec_t exec)
     return MU_OK;
 }
 
+static int emulate_into(mu_dag_t dag, const mu::EmuOpts &eo, long long *results, int nres)
+{
+    mu::EmuResult r;
+    std::string err;
+    mu::LowerOpts lo;
+    lo.trsm_max = std::min(lo.trsm_max, mu::PERSIST_TRSM_MAX);
+    if (!mu::emulate(dag->dag, lo, eo, r, err))
+        return fail(MU_EINVAL, err);
+    const long long v[9] = {(long long)r.executed, (long long)r.duplicates,
+                            (long long)r.order_violations, (long long)r.err,
+                            (long long)r.done, (long long)r.nitems,
+                            (long long)r.max_task_workgroups, (long long)r.task_cap,
+                            (long long)r.max_tasks_running};
+    for (int i = 0; i < nres; i++)
+        results[i] = v[i];
+    return MU_OK;
+}
+
 int mu_dag_emulate(mu_dag_t dag, int threads, double timeout_s, int bump_unit,
                    long long *results)
 {
     if (!dag || !results || threads < 1 || timeout_s <= 0)
         return fail(MU_EINVAL, "mu_dag_emulate: bad arguments");
-    mu::EmuResult r;
-    std::string err;
-    if (!mu::emulate(dag->dag, mu::LowerOpts(), threads, timeout_s, bump_unit, r, err))
-        return fail(MU_EINVAL, err);
-    results[0] = (long long)r.executed;
-    results[1] = (long long)r.duplicates;
-    results[2] = (long long)r.order_violations;
-    results[3] = (long long)r.err;
-    results[4] = (long long)r.done;
-    results[5] = (long long)r.nitems;
-    return MU_OK;
+    mu::EmuOpts eo;
+    eo.workgroups = threads;
+    eo.timeout_s = timeout_s;
+    eo.bump_unit = bump_unit;
+    return emulate_into(dag, eo, results, 6);
+}
+
+int mu_dag_emulate_ex(mu_dag_t dag, const mu_emu_opts_t *opts, long long *results)
+{
+    if (!dag || !opts || !results || opts->workgroups < 1 || opts->timeout_s <= 0 ||
+        opts->sm < 0 || opts->sm > 100 || opts->producer_delay_us < 0)
+        return fail(MU_EINVAL, "mu_dag_emulate_ex: bad arguments");
+    mu::EmuOpts eo;
+    eo.workgroups = opts->workgroups;
+    eo.timeout_s = opts->timeout_s;
+    eo.host_deadline_s = opts->host_deadline_s;
+    eo.bump_unit = opts->bump_unit;
+    eo.producer_delay_us = opts->producer_delay_us;
+    eo.sm = opts->sm;
+    return emulate_into(dag, eo, results, 9);
 }
 
 // ---- tiled factorizations -----------------------------------------------------

@@ -1,0 +1,561 @@
+// mu_runtime.cpp -- host side of the execution of lowered unit DAGs on one
+// gfx950 GPU: the tiled mode's hipGraph of level kernels, the stream mode,
+// and the persistent scheduler's state, launch and host watchdog.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+
+#include "mu_kernels.hpp"
+#include "mu_runtime.hpp"
+
+namespace mu {
+
+#define HIPCHK(x)                                                        \
+    do {                                                                 \
+        hipError_t e_ = (x);                                             \
+        if (e_ != hipSuccess) {                                          \
+            err = std::string(#x) + ": " + hipGetErrorString(e_);        \
+            return -1;                                                   \
+        }                                                                \
+    } while (0)
+
+static bool gemm_only_op(int op) { return op == OP_GEMM || op == OP_SYRK; }
+
+static bool small_only_op(int op) { return op == OP_GETRF || op == OP_POTRF || op == OP_CVT; }
+
+// Algorithmic flops of one lowered unit (the whole operation its items cover).
+static double unit_flops(const UnitDesc &u)
+{
+    const double m = u.m, n = u.n, k = u.k;
+    switch (u.op) {
+    case OP_GEMM: return 2.0 * m * n * k;
+    case OP_SYRK: return m * (m + 1) * k;
+    case OP_TRSM: return m * m * n;  // triangle of order m, n right-hand sides
+    case OP_GETRF: return 2.0 * m * m * m / 3.0;
+    case OP_POTRF: return m * m * m / 3.0;
+    default: return 0.0;
+    }
+}
+
+static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int run_units(const Lowered &low, hipStream_t s, std::string &err)
+{
+    // units of an executable come patched; an eager call gets a
+    // stream-ordered workspace for the life of its launches
+    const size_t wsd = ws_total_doubles(low);
+    bool patched = true;
+    for (const UnitDesc &d : low.units)
+        if (d.ws_slot >= 0 && ((d.op == OP_TRINV && !d.c) || (d.op == OP_TRSM && !d.b)))
+            patched = false;
+    if (wsd && !patched) {
+        double *ws = nullptr;
+        HIPCHK(hipMallocAsync((void **)&ws, wsd * sizeof(double), s));
+        std::vector<UnitDesc> units = low.units;
+        patch_workspace(units, low, ws);
+        Lowered copy;
+        copy.units = std::move(units);
+        int rc = run_units(copy, s, err);
+        hipError_t fe = hipFreeAsync(ws, s);
+        if (rc != 0)
+            return rc;
+        HIPCHK(fe);
+        return 0;
+    }
+    for (const UnitDesc &d : low.units) {
+        if (d.nitems <= 0)
+            continue;
+        if (launch_unit(d, s) != 0)
+            HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
+                      const ExecOpts &eopts, Exec **out, std::string &err);
+
+int exec_create(const Dag &dag, int mode, const LowerOpts &lopts, const ExecOpts &eopts,
+                Exec **out, std::string &err)
+{
+    Exec *e = new Exec();
+    e->mode = mode;
+    if (exec_build(e, dag, mode, lopts, eopts, out, err) != 0) {
+        exec_destroy(e);  // frees whatever the failed build had allocated
+        return -1;
+    }
+    return 0;
+}
+
+static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
+                      const ExecOpts &eopts, Exec **out, std::string &err)
+{
+    if (!lower(dag, lopts, e->low, err))
+        return -1;
+    Lowered &L = e->low;
+    const uint32_t nu = (uint32_t)L.units.size();
+    HIPCHK(hipGetDevice(&e->device));
+    if (const size_t wsd = ws_total_doubles(L)) {
+        HIPCHK(hipMalloc((void **)&e->ws, wsd * sizeof(double)));
+        patch_workspace(L.units, L, e->ws);
+    }
+
+    // CSR successors and per-priority ring capacities
+    std::vector<uint32_t> succ;
+    uint32_t cap[NPRIO] = {0, 0, 0, 0};
+    for (uint32_t u = 0; u < nu; u++) {
+        L.units[u].succ_begin = (uint32_t)succ.size();
+        succ.insert(succ.end(), L.succ[u].begin(), L.succ[u].end());
+        L.units[u].succ_end = (uint32_t)succ.size();
+        cap[L.units[u].prio] += (uint32_t)L.units[u].nitems;
+    }
+    std::vector<uint32_t> item_unit(L.nitems);
+    for (uint32_t u = 0; u < nu; u++)
+        for (int i = 0; i < L.units[u].nitems; i++)
+            item_unit[L.units[u].item_base + i] = u;
+
+    if (mode == MODE_GRAPH) {
+        // Level of each unit = longest chain of units before it; units of one
+        // level are independent.  Lane: a unit of a trailing update that
+        // feeds only further updates goes to the bulk lane 1, every unit on
+        // the way to the next panel to lane 0.  Levels increase along every
+        // unit edge, so the node graph below is acyclic, and the bulk update
+        // of one step runs beside the next step's panel work (lookahead).
+        std::vector<int> level(nu, 0);
+        int nlev = nu ? 1 : 0;
+        for (uint32_t u = 0; u < nu; u++)
+            for (uint32_t v : L.succ[u]) {
+                level[v] = std::max(level[v], level[u] + 1);
+                nlev = std::max(nlev, level[v] + 1);
+            }
+        // a unit belongs to a trailing update when its recorded task is a
+        // GEMM/SYRK task (not a piece of a nested factorization or solve) or
+        // the conversion of a panel tile to its reduced-precision shadow;
+        // such a unit whose successors are all update units is bulk work ...
+        auto update_task = [&](uint32_t u) {
+            const int tk = L.units[u].task;
+            if (tk < 0 || tk >= (int)dag.tasks.size() || dag.tasks[tk].ops.empty())
+                return false;
+            for (const OpRec &op : dag.tasks[tk].ops)
+                if (!gemm_only_op(op.op) && op.op != OP_CVT)
+                    return false;
+            return true;
+        };
+        // ... and so does every unit of a task no other task depends on (a
+        // result nothing in this graph reads, such as the inverse of a
+        // factored diagonal tile) whose successors stay inside that task
+        std::vector<char> sink_task(dag.tasks.size(), 1);
+        for (const TaskRec &tr : dag.tasks)
+            for (int d : tr.deps)
+                if (d >= 0 && d < (int)sink_task.size())
+                    sink_task[d] = 0;
+        std::vector<int> lane(nu, 0);
+        if (eopts.lookahead)
+            for (uint32_t u = 0; u < nu; u++) {
+                const int tk = L.units[u].task;
+                if (update_task(u)) {
+                    bool bulk = true;
+                    for (uint32_t v : L.succ[u])
+                        bulk = bulk && update_task(v);
+                    lane[u] = bulk ? 1 : 0;
+                } else if (tk >= 0 && tk < (int)sink_task.size() && sink_task[tk]) {
+                    bool inner = true;
+                    for (uint32_t v : L.succ[u])
+                        inner = inner && L.units[v].task == tk;
+                    lane[u] = inner ? 1 : 0;
+                }
+            }
+        // the kernel a unit's items run on: GEMM/SYRK blocks on k_batch_gemm,
+        // small diagonal factorizations / conversions on k_batch_small, panel
+        // solves with inverted diagonal blocks (and the inversions, TRINV) on
+        // the 4-wave k_batch_trsm_mid<256 / 512> or, for larger triangles, the
+        // 8-wave k_batch_trsm, small solves without inverses on
+        // k_batch_trsm_small; anything else on the generic k_batch
+        auto ukind = [&](const UnitDesc &d) -> int {
+            if (d.op == OP_GEMM && (d.flags & F_NARROW))
+                return LEVEL_GEMM_NARROW;
+            if (gemm_only_op(d.op))
+                return LEVEL_GEMM;
+            if (small_only_op(d.op))
+                return LEVEL_SMALL;
+            if (d.op == OP_TRINV)
+                return LEVEL_TRSM_256;
+            if (d.op == OP_TRSM && (d.flags & F_INV))
+                return d.m <= 256 ? LEVEL_TRSM_256 : d.m <= 512 ? LEVEL_TRSM_512 : LEVEL_TRSM;
+            if (d.op == OP_TRSM && d.m <= SMALL_MAX)
+                return LEVEL_TRSM_SMALL;
+            return LEVEL_GENERIC;
+        };
+        // one kernel node per (level, lane, kernel) with work, in that order
+        // (a topological order of the node graph).  The nodes of a lane form a
+        // chain of levels: each depends on every node of the lane's previous
+        // level; a unit whose predecessor sits in the other lane adds an edge
+        // between the two nodes.
+        const size_t NK = LEVEL_KINDS;
+        const size_t nkey = 2 * NK * (size_t)std::max(nlev, 1);
+        auto ukey = [&](uint32_t u) {
+            return (2 * (size_t)level[u] + lane[u]) * NK + ukind(L.units[u]);
+        };
+        std::vector<int> node_of(nkey, -1);
+        std::vector<std::vector<uint32_t>> lv;
+        std::vector<int> node_lane, node_level;
+        {
+            std::vector<std::vector<uint32_t>> by(nkey);
+            for (uint32_t u = 0; u < nu; u++)
+                for (int i = 0; i < L.units[u].nitems; i++)
+                    by[ukey(u)].push_back(L.units[u].item_base + i);
+            for (size_t k = 0; k < by.size(); k++)
+                if (!by[k].empty()) {
+                    node_of[k] = (int)lv.size();
+                    node_lane.push_back((int)((k / NK) & 1));
+                    node_level.push_back((int)(k / NK / 2));
+                    e->level_gemm.push_back((char)(k % NK));
+                    lv.push_back(std::move(by[k]));
+                }
+        }
+        const int nnode = (int)lv.size();
+        // node of a unit; units without items (none are lowered) map to -1
+        auto unode = [&](uint32_t u) { return node_of[ukey(u)]; };
+        std::vector<std::vector<int>> node_deps(nnode);
+        {
+            // per lane: the nodes of the last level seen, and of the level before
+            std::vector<int> prev[2], cur[2];
+            int cur_level[2] = {-1, -1};
+            for (int k = 0; k < nnode; k++) {
+                const int ln = node_lane[k];
+                if (node_level[k] != cur_level[ln]) {
+                    prev[ln] = cur[ln];
+                    cur[ln].clear();
+                    cur_level[ln] = node_level[k];
+                }
+                node_deps[k] = prev[ln];
+                cur[ln].push_back(k);
+            }
+        }
+        for (uint32_t u = 0; u < nu; u++)
+            for (uint32_t v : L.succ[u])
+                if (lane[u] != lane[v]) {
+                    const int a = unode(u), b = unode(v);
+                    if (a >= 0 && b >= 0 && a != b)
+                        node_deps[b].push_back(a);
+                }
+        for (auto &d : node_deps) {
+            std::sort(d.begin(), d.end());
+            d.erase(std::unique(d.begin(), d.end()), d.end());
+        }
+        std::vector<uint32_t> items;
+        std::vector<size_t> off;
+        for (auto &v : lv) {
+            off.push_back(items.size());
+            items.insert(items.end(), v.begin(), v.end());
+        }
+        size_t o = 0;
+        auto take = [&](size_t bytes) {
+            size_t r = align_up(o, 256);
+            o = r + bytes;
+            return r;
+        };
+        const size_t o_units = take(sizeof(UnitDesc) * std::max<uint32_t>(nu, 1));
+        const size_t o_iu = take(sizeof(uint32_t) * std::max<size_t>(L.nitems, 1));
+        const size_t o_items = take(sizeof(uint32_t) * std::max<size_t>(items.size(), 1));
+        e->dbytes = o;
+        HIPCHK(hipMalloc(&e->dbuf, e->dbytes));
+        if (nu)
+            HIPCHK(hipMemcpy(e->dbuf + o_units, L.units.data(), sizeof(UnitDesc) * nu,
+                             hipMemcpyHostToDevice));
+        if (L.nitems)
+            HIPCHK(hipMemcpy(e->dbuf + o_iu, item_unit.data(), sizeof(uint32_t) * L.nitems,
+                             hipMemcpyHostToDevice));
+        if (!items.empty())
+            HIPCHK(hipMemcpy(e->dbuf + o_items, items.data(), sizeof(uint32_t) * items.size(),
+                             hipMemcpyHostToDevice));
+        const UnitDesc *du = reinterpret_cast<const UnitDesc *>(e->dbuf + o_units);
+        const uint32_t *diu = reinterpret_cast<const uint32_t *>(e->dbuf + o_iu);
+        const uint32_t *ditems = reinterpret_cast<const uint32_t *>(e->dbuf + o_items);
+        e->level_flops.assign(nnode, 0.0);
+        for (uint32_t u = 0; u < nu; u++)
+            if (L.units[u].nitems > 0)
+                e->level_flops[unode(u)] += unit_flops(L.units[u]);
+        HIPCHK(hipGraphCreate(&e->graph, 0));
+        std::vector<hipGraphNode_t> gnodes(nnode);
+        for (int l = 0; l < nnode; l++) {
+            const uint32_t *first = ditems + off[l];
+            void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
+            hipKernelNodeParams kp;
+            std::memset(&kp, 0, sizeof kp);
+            kp.func = const_cast<void *>(level_kernel(e->level_gemm[l]));
+            kp.gridDim = dim3((unsigned)lv[l].size());
+            kp.blockDim = dim3((unsigned)level_threads(e->level_gemm[l]));
+            kp.kernelParams = args;
+            std::vector<hipGraphNode_t> deps;
+            for (int d : node_deps[l])
+                deps.push_back(gnodes[d]);
+            HIPCHK(hipGraphAddKernelNode(&gnodes[l], e->graph, deps.empty() ? nullptr : deps.data(),
+                                         deps.size(), &kp));
+        }
+        e->levels = nnode;
+        e->depth = nlev;
+        e->level_items = std::move(items);
+        e->level_off.assign(off.begin(), off.end());
+        e->level_off.push_back(e->level_items.size());
+        e->level_args[0] = (const void *)du;
+        e->level_args[1] = (const void *)diu;
+        e->level_args[2] = (const void *)ditems;
+        if (nnode)
+            HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+        *out = e;
+        return 0;
+    }
+    if (mode == MODE_STREAM) {
+        *out = e;
+        return 0;
+    }
+
+    // persistent mode: every unit must fit the persistent kernel's item
+    // bodies (the lowering options of this mode guarantee it)
+    for (uint32_t u = 0; u < nu; u++) {
+        const UnitDesc &d = L.units[u];
+        const bool ok = (d.op == OP_TRSM && (d.flags & F_INV)) ? d.m <= PERSIST_TRSM_MAX
+                        : d.op == OP_TRSM                      ? d.m <= SMALL_MAX
+                        : (d.op == OP_GETRF || d.op == OP_POTRF) ? d.m <= SMALL_MAX
+                                                                 : true;
+        if (!ok) {
+            err = "persistent mode: a unit exceeds the persistent kernel's item sizes "
+                  "(nest triangular solves at <= 256, factorizations at <= 128)";
+            return -1;
+        }
+    }
+    // one device allocation for descriptors and state
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = align_up(off, 256);
+        off = o + bytes;
+        return o;
+    };
+    const uint32_t ntasks = (uint32_t)std::max<size_t>(dag.tasks.size(), 1);
+    const size_t o_units = take(sizeof(UnitDesc) * nu);
+    const size_t o_iu = take(sizeof(uint32_t) * L.nitems);
+    const size_t o_succ = take(sizeof(uint32_t) * std::max<size_t>(succ.size(), 1));
+    const size_t o_dep = take(sizeof(int32_t) * nu);
+    const size_t o_left = take(sizeof(int32_t) * nu);
+    uint32_t total = 0;
+    for (int p = 0; p < NPRIO; p++)
+        total += cap[p];
+    const size_t o_slots = take(sizeof(uint32_t) * std::max<uint32_t>(total, 1));
+    const size_t o_init = take(sizeof(uint32_t) * std::max<uint32_t>(total, 1));
+    const size_t o_st = take(sizeof(SchedState));
+    const size_t o_run = take(sizeof(int32_t) * ntasks);
+    const size_t o_trace = eopts.trace ? take(sizeof(uint64_t) * 2 * L.nitems) : 0;
+    e->dbytes = off;
+    HIPCHK(hipMalloc(&e->dbuf, e->dbytes));
+    // the host-written abort word: pinned, mapped host memory
+    HIPCHK(hipHostMalloc((void **)&e->host_abort, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    __atomic_store_n(e->host_abort, 0u, __ATOMIC_SEQ_CST);
+    uint32_t *dev_abort = nullptr;
+    HIPCHK(hipHostGetDevicePointer((void **)&dev_abort, e->host_abort, 0));
+    HIPCHK(hipEventCreateWithFlags(&e->done_ev, hipEventDisableTiming));
+
+    // initial ring contents: items of the units without dependencies
+    std::vector<uint32_t> init(std::max<uint32_t>(total, 1), 0);
+    uint32_t ring_base[NPRIO], fill[NPRIO] = {0, 0, 0, 0};
+    for (int p = 0, b = 0; p < NPRIO; p++) {
+        ring_base[p] = b;
+        b += cap[p];
+    }
+    for (uint32_t u = 0; u < nu; u++) {
+        const UnitDesc &d = L.units[u];
+        if (d.ndeps == 0)
+            for (int i = 0; i < d.nitems; i++)
+                init[ring_base[d.prio] + fill[d.prio]++] = d.item_base + i + 1;
+    }
+    HIPCHK(hipMemcpy(e->dbuf + o_units, L.units.data(), sizeof(UnitDesc) * nu,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->dbuf + o_iu, item_unit.data(), sizeof(uint32_t) * L.nitems,
+                     hipMemcpyHostToDevice));
+    if (!succ.empty())
+        HIPCHK(hipMemcpy(e->dbuf + o_succ, succ.data(), sizeof(uint32_t) * succ.size(),
+                         hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->dbuf + o_init, init.data(), sizeof(uint32_t) * init.size(),
+                     hipMemcpyHostToDevice));
+
+    int ncu = 0, per_cu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, e->device));
+    if (persist_blocks_per_cu(&per_cu) != 0)
+        HIPCHK(hipGetLastError());
+    if (per_cu < 1)
+        per_cu = 1;
+    e->grid = eopts.grid > 0 ? eopts.grid : ncu * per_cu;
+
+    ExecView &ev = e->ev;
+    std::memset(&ev, 0, sizeof ev);
+    ev.units = reinterpret_cast<const UnitDesc *>(e->dbuf + o_units);
+    ev.item_unit = reinterpret_cast<const uint32_t *>(e->dbuf + o_iu);
+    ev.succ = reinterpret_cast<const uint32_t *>(e->dbuf + o_succ);
+    ev.dep_count = reinterpret_cast<int32_t *>(e->dbuf + o_dep);
+    ev.items_left = reinterpret_cast<int32_t *>(e->dbuf + o_left);
+    for (int p = 0; p < NPRIO; p++) {
+        ev.slots[p] = reinterpret_cast<uint32_t *>(e->dbuf + o_slots) + ring_base[p];
+        ev.cap[p] = cap[p];
+    }
+    e->slots_init = reinterpret_cast<uint32_t *>(e->dbuf + o_init);
+    e->slots_total = total;
+    e->ntasks = ntasks;
+    ev.st = reinterpret_cast<SchedState *>(e->dbuf + o_st);
+    ev.nunits = nu;
+    ev.nitems = (uint32_t)L.nitems;
+    e->trace = eopts.trace ? reinterpret_cast<uint64_t *>(e->dbuf + o_trace) : nullptr;
+    ev.trace = e->trace;
+    ev.host_abort = dev_abort;
+    ev.task_running = reinterpret_cast<int32_t *>(e->dbuf + o_run);
+    // CU budget per recorded task (--sm, percent of the GPU): workgroups per
+    // task, and the task's share in the running-task sum that admits tasks
+    // while it is below 100
+    if (eopts.sm > 0 && eopts.sm < 100) {
+        ev.task_budget = eopts.sm;
+        ev.task_cap = std::max(1, (int)(((long long)eopts.sm * e->grid + 99) / 100));
+    } else if (eopts.sm >= 100) {
+        ev.task_budget = 100;
+        ev.task_cap = e->grid;
+    }
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, e->device));
+    // the constant wall clock runs at 1 MHz .. 1 GHz; anything else is a
+    // misreported rate, and 100 MHz (the CDNA value) is used instead
+    e->wall_khz = (khz >= 1000 && khz <= 1000000) ? (uint64_t)khz : 100000;
+    ev.timeout_ticks = (uint64_t)(eopts.timeout_s * 1000.0 * (double)e->wall_khz);
+    // backstop independent of the clock: one empty poll takes well over 50 ns
+    ev.poll_limit = (uint64_t)(eopts.timeout_s * 2.0e7) + 1000;
+    e->host_timeout_s = 3.0 * eopts.timeout_s + 1.0;
+    std::memcpy(e->init_cnt, fill, sizeof fill);
+    *out = e;
+    return 0;
+}
+
+// The host watchdog of a persistent run: waits for the run's end event and,
+// past the deadline, sets the abort word every polling loop of the kernel
+// reads, so a run that stopped making progress always ends.
+static void watchdog_join(Exec *e)
+{
+    if (e->watchdog.joinable()) {
+        e->wd_stop.store(true);
+        e->watchdog.join();
+    }
+}
+
+int exec_launch(Exec *e, hipStream_t s, std::string &err)
+{
+    if (e->mode == MODE_GRAPH) {
+        if (e->gexec)
+            HIPCHK(hipGraphLaunch(e->gexec, s));
+        return 0;
+    }
+    if (e->mode == MODE_STREAM)
+        return run_units(e->low, s, err);
+    watchdog_join(e);
+    __atomic_store_n(e->host_abort, 0u, __ATOMIC_SEQ_CST);
+    e->host_aborted = false;
+    ResetArgs ra;
+    std::memset(&ra, 0, sizeof ra);
+    for (int p = 0; p < NPRIO; p++)
+        ra.init_cnt[p] = (int32_t)e->init_cnt[p];
+    ra.slots_init = e->slots_init;
+    ra.slots_flat = e->ev.slots[0];
+    ra.slots_total = e->slots_total;
+    ra.ntasks = e->ntasks;
+    if (persist_launch(e->ev, ra, e->grid, s) != 0)
+        HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->done_ev, s));
+    e->wd_stop.store(false);
+    const double deadline = e->host_timeout_s;
+    e->watchdog = std::thread([e, deadline]() {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            if (e->wd_stop.load())
+                return;
+            if (hipEventQuery(e->done_ev) != hipErrorNotReady)
+                return;
+            const double el =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el > deadline) {
+                e->host_aborted = true;
+                __atomic_store_n(e->host_abort, 1u, __ATOMIC_SEQ_CST);
+                return;
+            }
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        }
+    });
+    return 0;
+}
+
+int exec_launch_level(Exec *e, int level, hipStream_t s, std::string &err)
+{
+    if (e->mode != MODE_GRAPH || level < 0 || level >= e->levels) {
+        err = "exec_launch_level: tiled executables only, level out of range";
+        return -1;
+    }
+    const UnitDesc *du = (const UnitDesc *)e->level_args[0];
+    const uint32_t *diu = (const uint32_t *)e->level_args[1];
+    const uint32_t *first = (const uint32_t *)e->level_args[2] + e->level_off[level];
+    const unsigned n = (unsigned)(e->level_off[level + 1] - e->level_off[level]);
+    void *args[] = {(void *)&du, (void *)&diu, (void *)&first};
+    HIPCHK(hipLaunchKernel(level_kernel(e->level_gemm[level]), dim3(n),
+                           dim3((unsigned)level_threads(e->level_gemm[level])),
+                           args, 0, s));
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int exec_check(Exec *e, std::string &err)
+{
+    if (e->mode != MODE_PERSISTENT)
+        return 0;
+    HIPCHK(hipEventSynchronize(e->done_ev));
+    watchdog_join(e);
+    SchedState st;
+    HIPCHK(hipMemcpy(&st, e->ev.st, sizeof st, hipMemcpyDeviceToHost));
+    if (st.err || st.done != e->ev.nitems || e->host_aborted) {
+        const char *why = st.err == 1   ? "no progress for the time-out"
+                          : st.err == 2 ? "poll limit without progress"
+                          : st.err == 3 ? "more claims than queued items"
+                          : e->host_aborted ? "aborted by the host watchdog"
+                                            : "stopped early";
+        err = "persistent scheduler stopped after " + std::to_string(st.done) + " of " +
+              std::to_string(e->ev.nitems) + " items (" + why + ")";
+        return -1;
+    }
+    return 0;
+}
+
+int exec_trace(Exec *e, std::vector<uint64_t> &stamps, std::string &err)
+{
+    if (!e->trace) {
+        err = "tracing was not enabled";
+        return -1;
+    }
+    stamps.resize(2 * (size_t)e->ev.nitems);
+    HIPCHK(hipMemcpy(stamps.data(), e->trace, sizeof(uint64_t) * stamps.size(),
+                     hipMemcpyDeviceToHost));
+    return 0;
+}
+
+void exec_destroy(Exec *e)
+{
+    if (!e)
+        return;
+    watchdog_join(e);
+    if (e->done_ev)
+        (void)hipEventDestroy(e->done_ev);
+    if (e->gexec)
+        (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph)
+        (void)hipGraphDestroy(e->graph);
+    if (e->dbuf)
+        (void)hipFree(e->dbuf);
+    if (e->ws)
+        (void)hipFree(e->ws);
+    if (e->host_abort)
+        (void)hipHostFree(e->host_abort);
+    delete e;
+}
+
+}  // namespace mu

@@ -2,10 +2,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mu_dag.hpp"
+#include "mu_kernels.hpp"
 
 namespace mu {
 
@@ -17,22 +20,14 @@ enum Mode : int {
 
 struct ExecOpts {
     int grid = 0;              // persistent grid; 0 = one workgroup per resident slot
-    double timeout_s = 30.0;   // persistent: seconds without progress before abort
+    double timeout_s = 10.0;   // persistent: seconds without progress before abort (the host
+                               // watchdog aborts a run after 3 x timeout_s + 1 s)
     int trace = 0;             // persistent: record per-item wall-clock stamps
     int lookahead = 1;         // tiled: bulk updates on a second, concurrent chain
+    int sm = 0;                // persistent: CU budget per recorded task, percent of the
+                               // GPU (the reference's --sm); 0 = none
 };
 
-enum LevelKernel : char {
-    LEVEL_GENERIC = 0,
-    LEVEL_GEMM = 1,
-    LEVEL_SMALL = 2,
-    LEVEL_TRSM = 3,
-    LEVEL_TRSM_SMALL = 4,
-    LEVEL_TRSM_256 = 5,
-    LEVEL_TRSM_512 = 6,
-    LEVEL_GEMM_NARROW = 7,
-    LEVEL_KINDS = 8,
-};
 
 struct Exec {
     int mode = MODE_PERSISTENT;
@@ -45,10 +40,18 @@ struct Exec {
     ExecView ev{};
     uint32_t *slots_init = nullptr;  // initial queue contents, same layout as the rings
     uint32_t slots_total = 0;
-    uint32_t init_tail[NPRIO] = {0, 0, 0, 0};
+    uint32_t init_cnt[NPRIO] = {0, 0, 0, 0};
+    uint32_t ntasks = 0;
     uint64_t *trace = nullptr;
     int grid = 0;
     uint64_t wall_khz = 100000;
+    // host watchdog of a persistent run
+    uint32_t *host_abort = nullptr;  // pinned host word the kernel polls
+    hipEvent_t done_ev = nullptr;    // recorded after the persistent kernel
+    std::thread watchdog;
+    std::atomic<bool> wd_stop{false};
+    std::atomic<bool> host_aborted{false};
+    double host_timeout_s = 31.0;
     // graph mode
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;

@@ -1,25 +1,40 @@
 // mu_sched_host.cpp -- host emulation of the persistent scheduler.
 //
-// Runs the protocol of mu_sched.hpp on the CPU: each std::thread plays one
-// workgroup, items are not computed but checked (every item runs exactly
-// once, and only after every item of every predecessor unit has finished).
-// Used by the CPU test suite and by `mu_dag_emulate` for diagnosing DAGs.
+// Runs the worker loop of mu_sched.hpp (sched_worker, the same code the
+// gfx950 kernel runs) on the CPU: one std::thread plays one wave, four of
+// them one workgroup, and a workgroup barrier stands for s_barrier; the
+// wave-collective steps run on wave 0's thread, the per-thread release step
+// on every wave thread for its 64 lanes.  Items are not computed but checked:
+// every item runs exactly once, only after every item of every predecessor
+// unit has finished, and no task runs on more workgroups than its CU budget.
+// Used by the CPU test suite (also under ThreadSanitizer) and by
+// mu_dag_emulate.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "mu_dag.hpp"
 #include "mu_sched.hpp"
+#include "mu_sched_host.hpp"
 
 namespace mu {
 
+namespace {
+
+std::atomic<int> g_producer_delay_us{0};
+
 struct HostAtom {
+    static bool lane0() { return true; }
+    static uint32_t bcast(uint32_t v) { return v; }
     static uint32_t load(uint32_t *p) { return __atomic_load_n(p, __ATOMIC_RELAXED); }
     static uint32_t load_acquire(uint32_t *p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
+    static uint32_t load_system(uint32_t *p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
     static void store(uint32_t *p, uint32_t v) { __atomic_store_n(p, v, __ATOMIC_RELAXED); }
     static uint32_t add(uint32_t *p, uint32_t v)
     {
@@ -47,21 +62,110 @@ struct HostAtom {
             .count();
     }
     static void pause() { std::this_thread::yield(); }
+    // a producer stalled between reserving ring positions and storing them
+    static void producer_delay()
+    {
+        const int us = g_producer_delay_us.load(std::memory_order_relaxed);
+        if (us > 0)
+            std::this_thread::sleep_for(std::chrono::microseconds(us));
+    }
 };
 
-struct EmuResult {
-    uint64_t executed = 0;
-    uint64_t duplicates = 0;
-    uint64_t order_violations = 0;
-    uint32_t err = 0;
-    uint32_t done = 0;
-    uint32_t nitems = 0;
+// A reusable barrier for the threads of one workgroup (s_barrier).
+class WgBarrier {
+  public:
+    explicit WgBarrier(int n) : n_(n) {}
+    void wait()
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        const uint64_t gen = gen_;
+        if (++count_ == n_) {
+            count_ = 0;
+            gen_++;
+            cv_.notify_all();
+        } else {
+            cv_.wait(lk, [&] { return gen_ != gen; });
+        }
+    }
+
+  private:
+    std::mutex m_;
+    std::condition_variable cv_;
+    int n_, count_ = 0;
+    uint64_t gen_ = 0;
 };
 
-// `bump_unit` >= 0 adds one to that unit's initial dependency count (a
-// dependency that is never satisfied), to exercise the progress watchdog.
-bool emulate(const Dag &dag, const LowerOpts &lo, int threads, double timeout_s, int bump_unit,
-             EmuResult &res, std::string &err)
+struct HostWg {
+    explicit HostWg(int waves) : bar(waves) {}
+    WgBarrier bar;
+    int s_item = -1, s_last = 0;  // the workgroup's two LDS words
+};
+
+void raise_max(std::atomic<int32_t> &m, int32_t v)
+{
+    int32_t cur = m.load();
+    while (v > cur && !m.compare_exchange_weak(cur, v)) {
+    }
+}
+
+struct Check {
+    const std::vector<std::vector<uint32_t>> *preds;
+    int32_t *left;
+    std::vector<std::atomic<uint32_t>> *runs;
+    std::atomic<uint64_t> violations{0};
+    std::vector<std::atomic<int32_t>> *task_now;  // workgroups running a task's items now
+    std::atomic<int32_t> task_max{0};
+    std::atomic<int32_t> tasks_active{0};  // tasks with a workgroup running one of their items
+    std::atomic<int32_t> tasks_max{0};
+    int body_barriers = 2;
+};
+
+struct HostWave {
+    const ExecView &ev;
+    HostWg &wg;
+    Check &chk;
+    int w;
+    int wave() const { return w; }
+    void barrier() { wg.bar.wait(); }
+    void set_item(int v) { wg.s_item = v; }
+    int item() const { return wg.s_item; }
+    void set_last(int v) { wg.s_last = v; }
+    int last() const { return wg.s_last; }
+    void fence_release() { std::atomic_thread_fence(std::memory_order_release); }
+    template <class F>
+    void threads(F f)
+    {
+        for (uint32_t lane = 0; lane < 64; lane++)
+            f(64u * (uint32_t)w + lane, (uint32_t)WG_THREADS);
+    }
+    void run(uint32_t u, int item)
+    {
+        const int task = ev.units[u].task;
+        if (w == 0) {
+            for (uint32_t p : (*chk.preds)[u])
+                if (__atomic_load_n(&chk.left[p], __ATOMIC_ACQUIRE) != 0)
+                    chk.violations++;
+            (*chk.runs)[item]++;
+            if (task >= 0 && task < (int)chk.task_now->size()) {
+                const int32_t now = ++(*chk.task_now)[task];
+                raise_max(chk.task_max, now);
+                if (now == 1)
+                    raise_max(chk.tasks_max, ++chk.tasks_active);
+            }
+        }
+        // the item body's own barriers
+        for (int b = 0; b < chk.body_barriers; b++)
+            barrier();
+        if (w == 0 && task >= 0 && task < (int)chk.task_now->size())
+            if (--(*chk.task_now)[task] == 0)
+                --chk.tasks_active;
+    }
+};
+
+}  // namespace
+
+bool emulate(const Dag &dag, const LowerOpts &lo, const EmuOpts &eo, EmuResult &res,
+             std::string &err)
 {
     Lowered L;
     if (!lower(dag, lo, L, err))
@@ -84,7 +188,7 @@ bool emulate(const Dag &dag, const LowerOpts &lo, int threads, double timeout_s,
         for (int i = 0; i < L.units[u].nitems; i++)
             item_unit[L.units[u].item_base + i] = u;
 
-    uint32_t total = cap[0] + cap[1] + cap[2] + cap[3];
+    const uint32_t total = cap[0] + cap[1] + cap[2] + cap[3];
     std::vector<uint32_t> slots(std::max<uint32_t>(total, 1), 0);
     std::vector<int32_t> dep(nu), left(nu);
     SchedState st;
@@ -95,13 +199,20 @@ bool emulate(const Dag &dag, const LowerOpts &lo, int threads, double timeout_s,
         b += cap[p];
     }
     for (uint32_t u = 0; u < nu; u++) {
-        dep[u] = L.units[u].ndeps + ((int)u == bump_unit ? 1 : 0);
+        const int p = L.units[u].prio;
+        dep[u] = L.units[u].ndeps + ((int)u == eo.bump_unit ? 1 : 0);
         left[u] = L.units[u].nitems;
         if (dep[u] == 0)
             for (int i = 0; i < L.units[u].nitems; i++)
-                slots[base[L.units[u].prio] + st.tail[L.units[u].prio]++] =
-                    L.units[u].item_base + i + 1;
+                slots[base[p] + st.tail[p]++] = L.units[u].item_base + i + 1;
     }
+    for (int p = 0; p < NPRIO; p++)
+        st.cnt[p] = (int32_t)st.tail[p];
+    const size_t ntasks = std::max<size_t>(dag.tasks.size(), 1);
+    std::vector<int32_t> task_running(ntasks, 0);
+    uint32_t host_abort = 0;
+
+    const int wgs = std::max(eo.workgroups, 1);
     ExecView ev;
     std::memset(&ev, 0, sizeof ev);
     ev.units = L.units.data();
@@ -116,38 +227,68 @@ bool emulate(const Dag &dag, const LowerOpts &lo, int threads, double timeout_s,
     ev.st = &st;
     ev.nunits = nu;
     ev.nitems = (uint32_t)L.nitems;
-    ev.timeout_ticks = (uint64_t)(timeout_s * 1e9);
+    ev.timeout_ticks = (uint64_t)(eo.timeout_s * 1e9);
     ev.poll_limit = ~0ull;
+    ev.host_abort = &host_abort;
+    ev.task_running = task_running.data();
+    if (eo.sm > 0) {
+        ev.task_budget = std::min(eo.sm, 100);
+        ev.task_cap = std::max(1, (int)(((long long)ev.task_budget * wgs + 99) / 100));
+    }
 
     std::vector<std::atomic<uint32_t>> runs(L.nitems);
     for (auto &r : runs)
         r.store(0);
-    std::atomic<uint64_t> violations{0};
-    auto worker = [&]() {
-        for (;;) {
-            const int it = sched_pop<HostAtom>(ev);
-            if (it < 0)
-                return;
-            const uint32_t u = item_unit[it];
-            for (uint32_t p : preds[u])
-                if (__atomic_load_n(&left[p], __ATOMIC_ACQUIRE) != 0)
-                    violations++;
-            runs[it]++;
-            if (sched_item_done<HostAtom>(ev, u))
-                sched_release<HostAtom>(ev, u, 0, 1);
-            sched_count<HostAtom>(ev);
-        }
-    };
+    std::vector<std::atomic<int32_t>> task_now(ntasks);
+    for (auto &t : task_now)
+        t.store(0);
+    Check chk;
+    chk.preds = &preds;
+    chk.left = left.data();
+    chk.runs = &runs;
+    chk.task_now = &task_now;
+
+    g_producer_delay_us.store(eo.producer_delay_us);
+    const int waves = 4;
+    std::vector<HostWg *> wg;
+    for (int g = 0; g < wgs; g++)
+        wg.push_back(new HostWg(waves));
     std::vector<std::thread> pool;
-    for (int t = 0; t < std::max(threads, 1); t++)
-        pool.emplace_back(worker);
+    for (int g = 0; g < wgs; g++)
+        for (int w = 0; w < waves; w++)
+            pool.emplace_back([&, g, w]() {
+                HostWave hw{ev, *wg[g], chk, w};
+                sched_worker<HostAtom>(ev, hw);
+            });
+    // the host side of the watchdog: past the deadline, set the abort word
+    std::atomic<bool> finished{false};
+    std::thread dog([&]() {
+        const auto t0 = std::chrono::steady_clock::now();
+        while (!finished.load()) {
+            const double el =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (eo.host_deadline_s > 0 && el > eo.host_deadline_s) {
+                __atomic_store_n(&host_abort, 1u, __ATOMIC_RELEASE);
+                return;
+            }
+            std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        }
+    });
     for (auto &t : pool)
         t.join();
+    finished.store(true);
+    dog.join();
+    g_producer_delay_us.store(0);
+    for (HostWg *p : wg)
+        delete p;
     res = EmuResult();
     res.nitems = (uint32_t)L.nitems;
     res.done = st.done;
-    res.err = st.err;
-    res.order_violations = violations.load();
+    res.err = st.err ? st.err : (__atomic_load_n(&host_abort, __ATOMIC_ACQUIRE) ? 4u : 0u);
+    res.order_violations = chk.violations.load();
+    res.max_task_workgroups = (uint32_t)chk.task_max.load();
+    res.task_cap = (uint32_t)ev.task_cap;
+    res.max_tasks_running = (uint32_t)chk.tasks_max.load();
     for (auto &r : runs) {
         const uint32_t c = r.load();
         res.executed += c ? 1 : 0;

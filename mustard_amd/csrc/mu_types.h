@@ -81,13 +81,17 @@ static_assert(sizeof(UnitDesc) == 128, "UnitDesc must stay 128 bytes");
 
 // Per-run mutable scheduler state, all in device memory.
 struct SchedState {
-    uint32_t head[NPRIO];
-    uint32_t tail[NPRIO];
-    uint32_t done;        // completed items
-    uint32_t abort;       // set on time-out; every workgroup exits
-    uint32_t err;         // error code (1 = time-out)
-    uint32_t pad_[5];
+    int32_t cnt[NPRIO];   // queued, unclaimed items per priority ring
+    uint32_t head[NPRIO]; // next ring position to hand out
+    uint32_t tail[NPRIO]; // next ring position to reserve
+    uint32_t done;        // items of completed units
+    uint32_t abort;       // set on a time-out or a host abort; every workgroup exits
+    uint32_t err;         // 1 = no progress for the time-out, 2 = poll limit, 3 = broken DAG,
+                          // 4 = aborted by the host
+    int32_t occ;          // CU budget of the running tasks, percent of the GPU (FLAGS_OCCUP)
+    uint32_t pad_[4];
 };
+static_assert(sizeof(SchedState) == 80, "SchedState layout");
 
 // Everything the device scheduler needs, passed by value to the kernels.
 struct ExecView {
@@ -104,6 +108,10 @@ struct ExecView {
     uint64_t timeout_ticks;        // wall-clock ticks without progress before abort
     uint64_t poll_limit;           // empty polls without progress before abort
     uint64_t *trace;               // optional: per item (start, end) wall-clock stamps
+    uint32_t *host_abort;          // host-written abort word (pinned host memory), or null
+    int32_t *task_running;         // per recorded task: workgroups running its items
+    int32_t task_cap;              // CU budget: workgroups per task (0 = no budget)
+    int32_t task_budget;           // ... and the task's share of the GPU in percent
 };
 
 }  // namespace mu

@@ -60,16 +60,58 @@ def _emulate(d, threads, timeout=20.0, bump=-1):
     return list(r)
 
 
-@pytest.mark.parametrize("n,t,kind", [(600, 2, "lu"), (600, 6, "chol"), (3000, 3, "lu"),
-                                      (24000, 24, "lu"), (24000, 24, "chol"), (4000, 2, "lu")])
-@pytest.mark.parametrize("threads", [1, 7, 48])
+@pytest.mark.parametrize("n,t,kind,threads", [
+    (600, 2, "lu", 1), (600, 2, "lu", 7), (600, 6, "chol", 48), (3000, 3, "lu", 7),
+    (3000, 3, "lu", 48), (4000, 2, "lu", 7), (6000, 6, "chol", 7), (6000, 6, "lu", 48)])
 def test_scheduler_protocol_emulated(n, t, kind, threads):
     """Every item runs exactly once, after all items of its predecessor units,
-    and the run terminates (the device protocol, run by host threads)."""
+    and the run terminates: the kernel's own worker loop (mu_sched.hpp), run
+    with four host threads per emulated workgroup and a barrier each."""
     d = mu.Dag()
     (d.record_lu if kind == "lu" else d.record_cholesky)(_fake(), n, t)
     executed, dup, viol, err, done, total = _emulate(d, threads)
     assert (executed, dup, viol, err, done) == (total, 0, 0, 0, total)
+
+
+def _emulate_ex(d, workgroups, timeout=20.0, deadline=0.0, bump=-1, delay_us=0, sm=0):
+    o = _lib.EmuOpts(workgroups, timeout, deadline, bump, delay_us, sm)
+    r = (ctypes.c_longlong * 9)()
+    mu.call("mu_dag_emulate_ex", d.h, ctypes.byref(o), r)
+    return list(r)
+
+
+def test_scheduler_survives_stalled_producers():
+    """A producer that stalls between reserving ring positions and storing
+    the item ids: the consumers that claimed those positions wait for them
+    (they never give a claimed position up), and every item runs once."""
+    d = mu.Dag().record_lu(_fake(), 3000, 3)
+    r = _emulate_ex(d, 12, delay_us=300)
+    executed, dup, viol, err, done, total = r[:6]
+    assert (executed, dup, viol, err, done) == (total, 0, 0, 0, total)
+
+
+@pytest.mark.parametrize("sm", [5, 20, 50, 100])
+def test_scheduler_cu_budget(sm):
+    """--sm (FLAGS_OCCUP): at most ceil(sm% of the workgroups) run one task's
+    items at once, and tasks start only while their budgets sum to less than
+    100, so sm=100 runs one task at a time."""
+    d = mu.Dag().record_cholesky(_fake(), 2000, 4)
+    r = _emulate_ex(d, 16, sm=sm)
+    executed, dup, viol, err, done, total, most, cap, tasks = r
+    assert (executed, dup, viol, err, done) == (total, 0, 0, 0, total)
+    assert cap == -(-sm * 16 // 100) and most <= cap
+    assert tasks <= (100 + sm - 1) // sm
+    if sm == 100:
+        assert tasks == 1
+
+
+def test_scheduler_host_watchdog_ends_a_stalled_run():
+    """A dependency that never resolves and a device time-out longer than the
+    host deadline: the host sets the abort word and every worker leaves."""
+    d = mu.Dag().record_lu(_fake(), 600, 2)
+    r = _emulate_ex(d, 4, timeout=600.0, deadline=0.3, bump=7)
+    executed, dup, viol, err, done, total = r[:6]
+    assert err == 4 and done < total and dup == 0 and viol == 0
 
 
 def test_scheduler_watchdog_detects_deadlock():
@@ -126,13 +168,3 @@ def test_capture_records_instead_of_running():
     assert n.value == 2 and list(ops[:2]) == [4, 1]
     with pytest.raises(mu.MustardError):
         d.end_capture(key)
-
-
-def test_persistent_mode_is_switched_off():
-    """--subgraph (the persistent scheduler) is refused with a clear error in
-    this build rather than run (see tests/test_gpu_tiled.py)."""
-    d = mu.Dag().record_lu(_fake(), 600, 2)
-    with pytest.raises(mu.MustardError) as e:
-        mu.Exec(d, "subgraph")
-    assert e.value.code == _lib.MU_ENOTSUP
-    assert "switched off" in str(e.value)

@@ -52,10 +52,18 @@ def test_single_kernel_mode_and_argh_space_form():
     assert "Result passes verification: 1" in out
 
 
-def test_subgraph_mode_is_refused():
-    rc, out, err = run("lu_mustard", "-n=600", "-t=2", "--subgraph")
-    assert rc == 1
-    assert "switched off" in err
+@pytest.mark.parametrize("prog", ["lu_mustard", "cholesky_mustard"])
+@pytest.mark.parametrize("sm", ["20", "100"])
+def test_subgraph_mode_runs_and_verifies(prog, sm):
+    """--subgraph: one persistent scheduler launch per run, the reference's
+    output lines, and --verify passes."""
+    rc, out, err = run(prog, "-n=600", "-t=2", "--subgraph", "--verify", "-r=2", f"--sm={sm}")
+    assert rc == 0, err
+    lines = out.splitlines()
+    assert lines[0] == "SUBGRAPH with N=600 (2 of 300x300 tiles)"
+    assert f"SM Limit per kernel = {sm}" in lines
+    assert sum(1 for l in lines if re.match(r"device 0 \| \d+ run \| time \(s\): ", l)) == 2
+    assert "Result passes verification: 1" in out
 
 
 def test_verbose_prints_totalnodes(oracle):

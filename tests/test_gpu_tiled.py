@@ -15,16 +15,7 @@ from tests.util import close_rel, spd
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["tiled", "stream"]
-
-# The persistent device scheduler did not terminate in its first two runs on
-# the MI355X (LU N=600, 2x2 tiles); its cause is not found yet, so the mode is
-# switched off in the library (mu_exec_create returns MU_ENOTSUP) and its GPU
-# tests are skipped.  The queue/counter protocol itself is tested on the CPU
-# (tests/test_capi.py, mu_dag_emulate).
-PERSISTENT_OFF = pytest.mark.skip(
-    reason="persistent scheduler switched off: first MI355X runs did not terminate, cause "
-           "not found yet (protocol covered by the CPU emulation tests)")
+MODES = ["tiled", "stream", "subgraph"]
 
 
 def _mu():
@@ -91,7 +82,7 @@ def test_tiled_vs_oracle(torch_gpu, oracle, kind, n, t):
         assert oracle.tiled_lu(ref, t) == 0
     else:
         assert oracle.tiled_cholesky(ref, t) == 0
-    for mode in (["tiled", "stream"] if n <= 3000 else ["tiled"]):
+    for mode in (["tiled", "stream", "subgraph"] if n <= 3000 else ["tiled", "subgraph"]):
         F, info, res, _ = _run(torch_gpu, kind, A, t, mode)
         assert info == 0
         if kind == "lu":
@@ -102,7 +93,6 @@ def test_tiled_vs_oracle(torch_gpu, oracle, kind, n, t):
         assert res <= 10.0, mode
 
 
-@PERSISTENT_OFF
 @pytest.mark.parametrize("grid", [1, 3, 64])
 def test_persistent_small_grids(torch_gpu, oracle, grid):
     """The scheduler is correct with any number of workgroups, down to one."""
@@ -115,7 +105,6 @@ def test_persistent_small_grids(torch_gpu, oracle, grid):
     assert close_rel(F, ref) <= 1.0
 
 
-@PERSISTENT_OFF
 def test_persistent_trace(torch_gpu, oracle):
     mu = _mu()
     n, t = 1200, 4
@@ -141,7 +130,6 @@ def test_persistent_trace(torch_gpu, oracle):
             assert last[p] <= first[task]
 
 
-@PERSISTENT_OFF
 def test_persistent_timeout_aborts_cleanly(torch_gpu, oracle):
     """A zero time-out makes idle workgroups abort; every workgroup exits and
     the run reports MU_ERUN instead of hanging."""
@@ -150,6 +138,58 @@ def test_persistent_timeout_aborts_cleanly(torch_gpu, oracle):
     dA = mu.to_device(A)
     dag = mu.Dag().record_lu(mu.ptr(dA), 600, 6)
     ex = mu.Exec(dag, "subgraph", grid=8, timeout_s=1e-9)
+    ex.launch()
+    torch_gpu.cuda.synchronize()
+    with pytest.raises(mu.MustardError) as e:
+        ex.check()
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("sm", [100, 20, 5])
+def test_persistent_cu_budget(torch_gpu, oracle, sm):
+    """--sm: each recorded task runs on at most sm% of the workgroups, and a
+    task starts only while the running tasks' budgets sum to less than 100
+    (at sm=100 the tasks run one at a time); the factors are the oracle's."""
+    mu = _mu()
+    n, t = 2000, 4
+    A = oracle.gen_ref(n)
+    ref = A.copy(order="F")
+    oracle.tiled_lu(ref, t)
+    dA = mu.to_device(A)
+    info = mu.info_cell()
+    dag = mu.Dag().record_lu(mu.ptr(dA), n, t, None, mu.ptr(info))
+    ex = mu.Exec(dag, "subgraph", trace=True, sm=sm)
+    ex.launch()
+    torch_gpu.cuda.synchronize()
+    ex.check()
+    assert mu.info_value(info) == 0
+    assert close_rel(mu.to_host(dA), ref) <= 1.0
+    cap = -(-sm * ex.stats()["grid"] // 100)
+    ev = []
+    for s_, e_, _, task, _ in ex.trace():
+        ev += [(s_, 1, task), (e_, -1, task)]
+    running, tasks = {}, 0
+    for _, d, task in sorted(ev, key=lambda x: (x[0], x[1])):
+        running[task] = running.get(task, 0) + d
+        assert running[task] <= cap
+        tasks = sum(1 for v in running.values() if v > 0)
+        if sm == 100:
+            assert tasks <= 1
+
+
+def test_persistent_host_watchdog(torch_gpu, oracle):
+    """A run whose device time-out is longer than the host deadline is ended
+    by the host: the abort word in pinned memory is set after 3 x timeout_s
+    + 1 s, every polling loop reads it, and the run reports MU_ERUN.  The
+    condition is forced from the host: a tiny grid on a long run with a
+    device time-out the run never reaches."""
+    mu = _mu()
+    n, t = 8000, 8
+    dA = mu.to_device(oracle.gen_ref(n))
+    dag = mu.Dag().record_lu(mu.ptr(dA), n, t)
+    # one workgroup: about 4.5 s of work (one CU runs some 77 GFLOP/s), four
+    # times the host deadline of 3 * 1e-3 + 1 s
+    ex = mu.Exec(dag, "subgraph", grid=1, timeout_s=1e-3)
     ex.launch()
     torch_gpu.cuda.synchronize()
     with pytest.raises(mu.MustardError) as e:
