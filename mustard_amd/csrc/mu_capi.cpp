@@ -522,8 +522,11 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
         eo.sm = opts->sm;
     }
     // the persistent kernel runs panel solves of order <= 256 (larger ones nest)
-    if (mode == MU_MODE_SUBGRAPH)
+    // and GEMM items of 2 x 2 output blocks
+    if (mode == MU_MODE_SUBGRAPH) {
         lo.trsm_max = std::min(lo.trsm_max, mu::PERSIST_TRSM_MAX);
+        lo.gemm_group = 2;
+    }
     for (const auto &t : dag->dag.tasks)
         for (const auto &op : t.ops)
             if (op.flags & (mu::F_LOW_F32 | mu::F_LOW_BF16))
@@ -688,6 +691,7 @@ static int emulate_into(mu_dag_t dag, const mu::EmuOpts &eo, long long *results,
     std::string err;
     mu::LowerOpts lo;
     lo.trsm_max = std::min(lo.trsm_max, mu::PERSIST_TRSM_MAX);
+    lo.gemm_group = 2;
     if (!mu::emulate(dag->dag, lo, eo, r, err))
         return fail(MU_EINVAL, err);
     const long long v[9] = {(long long)r.executed, (long long)r.duplicates,

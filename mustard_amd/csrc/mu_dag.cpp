@@ -79,7 +79,8 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 int unit_items(const UnitDesc &u)
 {
     switch (u.op) {
-    case OP_GEMM: return u.gm * u.gn;
+    case OP_GEMM:
+        return u.grp > 1 ? cdiv(u.gm, u.grp) * cdiv(u.gn, u.grp) : u.gm * u.gn;
     case OP_SYRK: return u.gm * (u.gm + 1) / 2;
     case OP_TRSM: return u.gn;
     case OP_GETRF:
@@ -298,6 +299,7 @@ struct Builder {
             if (pair_aligned(op.a, op.flags) && pair_aligned(op.b, op.flags) && op.lda % 2 == 0 &&
                 op.ldb % 2 == 0 && op.m % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;
+            d.grp = o.gemm_group > 1 ? o.gemm_group : 1;
             int u = add_unit(d);
             f.src = {u};
             f.snk = {u};

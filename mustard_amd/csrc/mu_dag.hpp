@@ -67,6 +67,8 @@ struct LowerOpts {
     int trsm_max = TRSM_NEST;   // triangular solves above this nest into blocks (<= TRSM_NMAX)
     int trsm_inv_min = SMALL_MAX + 1;  // triangles of at least this order use inverted diagonal blocks
     int nprio = NPRIO;
+    int gemm_group = 1;  // GEMM items of gemm_group x gemm_group output blocks (the persistent
+                         // kernel pays its per-item scheduling and ordering once per group)
 };
 
 struct Lowered {

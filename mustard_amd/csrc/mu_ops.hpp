@@ -1184,12 +1184,25 @@ __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDe
         gemm_dispatch_vec<false, true>(u, bi, bj, bi == bj, smem);
         return;
     }
-    const int bi = it % u.gm, bj = it / u.gm;
-    switch (u.flags & (F_TA | F_TB)) {
-    case 0: gemm_dispatch_vec<false, false>(u, bi, bj, false, smem); break;
-    case F_TB: gemm_dispatch_vec<false, true>(u, bi, bj, false, smem); break;
-    case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
-    default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
+    // an item covers g x g output blocks, computed one after the other (a
+    // barrier between them: the next block's staging overwrites the LDS
+    // the previous one read last)
+    const int g = u.grp > 1 ? u.grp : 1;
+    const int gm = (u.gm + g - 1) / g;
+    const int bi0 = (it % gm) * g, bj0 = (it / gm) * g;
+#pragma unroll 1
+    for (int q = 0; q < g * g; q++) {
+        const int bi = bi0 + q % g, bj = bj0 + q / g;
+        if (bi >= u.gm || bj >= u.gn)
+            continue;
+        if (q)
+            lds_barrier();
+        switch (u.flags & (F_TA | F_TB)) {
+        case 0: gemm_dispatch_vec<false, false>(u, bi, bj, false, smem); break;
+        case F_TB: gemm_dispatch_vec<false, true>(u, bi, bj, false, smem); break;
+        case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
+        default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
+        }
     }
 }
 

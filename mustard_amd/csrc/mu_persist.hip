@@ -128,9 +128,11 @@ struct DevWave {
     {
         return (int)__builtin_amdgcn_readfirstlane(*s_last);
     }
+    // this wave's stores have completed (reached the L2); the L2 write-back
+    // to device scope is wave 0's release that follows the barrier
     __device__ __forceinline__ void fence_release() const
     {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     }
     template <class F>
     __device__ __forceinline__ void threads(F f) const

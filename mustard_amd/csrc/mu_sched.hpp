@@ -135,16 +135,19 @@ MU_HD void sched_push_unit(const ExecView &ev, uint32_t u)
 }
 
 // Next item for this workgroup (wave-collective), or -1 when the run is
-// over or aborted.  Rings are tried in priority order.
+// over or aborted.  Rings are tried in priority order.  The fast path (an
+// item is queued) reads only the ring counters; the abort words -- one of
+// them in host memory, a PCIe round trip away -- are read on every 16th call
+// (`calls` counts them for the caller) and whenever nothing is queued.
 template <class AT>
-MU_HD int sched_pop(const ExecView &ev)
+MU_HD int sched_pop(const ExecView &ev, uint32_t &calls)
 {
-    uint64_t t0 = AT::now();
-    uint32_t seen = w_load<AT>(&ev.st->done);
-    uint64_t idle = 0;
+    if ((calls++ & 15u) == 0 && sched_stopped<AT>(ev))
+        return -1;
+    uint64_t t0 = 0, idle = 0;
+    uint32_t seen = 0;
+    bool first = true;
     for (;;) {
-        if (sched_stopped<AT>(ev))
-            return -1;
         for (int p = 0; p < NPRIO; p++) {
             uint32_t *cnt = reinterpret_cast<uint32_t *>(&ev.st->cnt[p]);
             if ((int32_t)w_load<AT>(cnt) <= 0)
@@ -158,6 +161,7 @@ MU_HD int sched_pop(const ExecView &ev)
                 sched_abort<AT>(ev, 3u);
                 return -1;
             }
+            // the producer that reserved `pos` is running and storing it
             for (;;) {
                 const uint32_t v = w_load_acquire<AT>(ev.slots[p] + pos);
                 if (v)
@@ -167,11 +171,15 @@ MU_HD int sched_pop(const ExecView &ev)
                 AT::pause();
             }
         }
+        // nothing queued: stop, finish, or wait with the watchdog armed
+        if (sched_stopped<AT>(ev))
+            return -1;
         const uint32_t d = w_load<AT>(&ev.st->done);
         if (d >= ev.nitems)
             return -1;
         const uint64_t now = AT::now();
-        if (d != seen) {
+        if (first || d != seen) {
+            first = false;
             seen = d;
             t0 = now;
             idle = 0;
@@ -263,8 +271,9 @@ MU_HD void sched_release(const ExecView &ev, uint32_t u, uint32_t first, uint32_
 template <class AT, class W>
 MU_HD void sched_worker(const ExecView &ev, W &w)
 {
+    uint32_t calls = 0;  // sched_pop calls of this workgroup (wave 0)
     if (w.wave() == 0) {
-        int it = sched_pop<AT>(ev);
+        int it = sched_pop<AT>(ev, calls);
         if (it >= 0 && !sched_admit<AT>(ev, ev.units[ev.item_unit[it]].task))
             it = -1;
         w.set_item(it);
@@ -276,7 +285,10 @@ MU_HD void sched_worker(const ExecView &ev, W &w)
             break;
         const uint32_t u = ev.item_unit[item];
         w.run(u, item);
-        w.fence_release();  // the item's writes, device-wide, before it counts as done
+        // every wave's writes are complete before the barrier; wave 0's
+        // release (the acq_rel decrement below) then makes them visible
+        // device-wide before the item counts as done
+        w.fence_release();
         w.barrier();
         if (w.wave() == 0)
             w.set_last(sched_item_done<AT>(ev, u) ? 1 : 0);
@@ -291,7 +303,7 @@ MU_HD void sched_worker(const ExecView &ev, W &w)
             sched_retire<AT>(ev, ev.units[u].task);
             if (last)
                 w_add_release<AT>(&ev.st->done, (uint32_t)ev.units[u].nitems);
-            int it = sched_pop<AT>(ev);
+            int it = sched_pop<AT>(ev, calls);
             if (it >= 0 && !sched_admit<AT>(ev, ev.units[ev.item_unit[it]].task))
                 it = -1;
             w.set_item(it);

@@ -75,7 +75,7 @@ struct UnitDesc {
     double *c;
     int32_t *info;         // LAPACK-style info cell (atomic min of failing column), or null
     int32_t ws_slot;       // TRSM with F_INV / TRINV: workspace slot of the inverted blocks, or -1
-    int32_t pad_;
+    int32_t grp;           // GEMM: an item covers grp x grp output blocks (0 or 1: one block)
 };
 static_assert(sizeof(UnitDesc) == 128, "UnitDesc must stay 128 bytes");
 

@@ -145,6 +145,7 @@ int main()
             record_cholesky(d, c.n, c.t);
         mu::LowerOpts lo;
         lo.trsm_max = 256;
+        lo.gemm_group = 2;
         mu::EmuOpts eo;
         eo.workgroups = c.wgs;
         eo.timeout_s = 60.0;
