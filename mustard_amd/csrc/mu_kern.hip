@@ -27,14 +27,18 @@ __global__ MU_GEMM_KERNEL void k_batch_gemm(const UnitDesc *units, const uint32_
     run_gemm_item(d, (int)(item - d.item_base), smem);
 }
 
-__global__ __launch_bounds__(WG_THREADS) void k_batch_small(const UnitDesc *units,
-                                                            const uint32_t *item_unit,
-                                                            const uint32_t *items)
+// Diagonal GETRF/POTRF (and precision conversions), 4 waves held to 168
+// registers per lane: a CU running three GEMM workgroups (104 registers per
+// wave) still has room for it, so the diagonal factorization of the next step
+// starts as soon as one bulk GEMM workgroup leaves a CU instead of waiting for
+// the bulk update to drain.
+__global__ __launch_bounds__(WG_THREADS) __attribute__((amdgpu_waves_per_eu(3, 3))) void
+k_batch_small(const UnitDesc *units, const uint32_t *item_unit, const uint32_t *items)
 {
     __shared__ double smem[LDS_SMALL];
     const uint32_t item = items[blockIdx.x];
     const UnitDesc d = units[item_unit[item]];
-    run_small_item(d, (int)(item - d.item_base), smem);
+    run_small_item<4>(d, (int)(item - d.item_base), smem);
 }
 
 // GEMMs with at most 16 output columns (substitution steps of a solve).
@@ -91,7 +95,10 @@ const void *level_kernel(int kind)
     }
 }
 
-int level_threads(int kind) { return kind == LEVEL_TRSM ? 64 * TRSM_KERNEL_WAVES : WG_THREADS; }
+int level_threads(int kind)
+{
+    return kind == LEVEL_TRSM ? 64 * TRSM_KERNEL_WAVES : WG_THREADS;
+}
 
 int launch_unit(const UnitDesc &d, hipStream_t s)
 {

@@ -1012,10 +1012,260 @@ __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc
     small_store(T, a, lda, n, ti, tj, true);
 }
 
+// ---------------------------------------------------------------------------
+// Blocked diagonal factorizations of an n x n tile (n <= SMALL_MAX), right-
+// looking over panels of 16 columns: no-pivot LU (GETRF) and lower Cholesky
+// (POTRF).  The tile sits in registers in the FP64 MFMA result layout --
+// 16 x 16 blocks, block-column j held by wave j % NW, so with 4 waves a
+// lane holds 2 block-columns x 8 blocks x 4 doubles and with 8 waves one
+// (element e of lane l of a block: row 4e + l/16, column l%16).  Panel p:
+//   * its owner wave (p % NW) copies block-column p to LDS and factors the
+//     panel alone, one lane per row (rows l and l + 64), the pivot row (LU)
+//     or the column's entries of the diagonal block (Cholesky) broadcast by
+//     readlane -- no barrier inside the panel; for LU it also forms the
+//     inverse of the unit lower 16 x 16 diagonal block;
+//   * after one barrier every wave updates its block-columns right of p on
+//     MFMA: LU forms U12 = inv(L11) A12 (four MFMAs; the result layout is
+//     the B operand of the next MFMAs) and A22 -= L21 U12; Cholesky
+//     A22 -= L21 L21^T on the lower blocks; L21 is read from the panel in
+//     LDS as the A operand;
+//   * a second barrier frees the panel buffer.
+// Rows and columns past n are the identity, so the padded factorization is
+// the n x n one; only the n x n part (Cholesky: its lower triangle) is
+// stored.  Zero pivots / non-positive minors report LAPACK-style info and
+// the factorization continues (the factors then hold inf / nan).
+// ---------------------------------------------------------------------------
+constexpr int BF_LD = 17;                          // LDS row stride of the panel
+constexpr int LDS_SMALL_PANEL = SMALL_MAX * BF_LD;  // 128 x 16 panel (17 KB)
+constexpr int LDS_SMALL_BLK = LDS_SMALL_PANEL + 16 * BF_LD;  // + inv(L11) for LU
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <bool CHOL, int NW = 4>
+__device__ __attribute__((always_inline)) inline void factor_small_blk(const UnitDesc &u,
+                                                                       double *smem)
+{
+    // NW waves (4 or 8); block-column j is held by wave j % NW
+    static_assert(NW == 4 || NW == 8, "4 or 8 waves");
+    constexpr int JJ = 8 / NW;
+    const int n = u.m, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int q = lane >> 4, cl = lane & 15;
+    const int nb = (n + 15) >> 4;
+    MU_G double *a = gp(u.c);
+    const size_t lda = u.ldc;
+    double *P = smem;                     // panel: row r (0..127), column c at P[r * 17 + c]
+    double *Li = smem + LDS_SMALL_PANEL;  // LU: inv(L11), row-major, stride 17
+    // the diagonal factorization is the serial chain of the whole DAG: its
+    // waves issue ahead of the GEMM waves sharing their SIMDs
+    __builtin_amdgcn_s_setprio(3);
+
+    v4d R[JJ][8];  // block (i, w + NW jj)
+#pragma unroll
+    for (int jj = 0; jj < JJ; jj++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = 16 * i + 4 * e + q, c = 16 * (w + NW * jj) + cl;
+                double v = r == c ? 1.0 : 0.0;
+                if (r < n && c < n && (!CHOL || r >= c))
+                    v = a[r + (size_t)c * lda];
+                R[jj][i][e] = v;
+            }
+
+    for (int p = 0; p < nb; p++) {
+        const int jp = p / NW;  // the owner's register column of block-column p
+        if (w == p % NW) {
+            // ---- panel: block-column p, rows 16p .. 127, to LDS ----
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (i >= p) {
+                    const v4d blk = (JJ > 1 && jp) ? R[JJ - 1][i] : R[0][i];
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        P[(16 * i + 4 * e + q) * BF_LD + cl] = blk[e];
+                }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            const int p0 = 16 * p;
+            // sweep 1: rows p0 .. p0 + 63, lane l on row p0 + l; the pivot
+            // row of column c is lane c's
+            {
+                const int r = p0 + lane;
+                const bool in = r < SMALL_MAX;
+                double x[16];
+#pragma unroll
+                for (int c = 0; c < 16; c++)
+                    x[c] = in ? P[r * BF_LD + c] : 0.0;
+#pragma unroll
+                for (int c = 0; c < 16; c++) {
+                    const double piv = readlane_d(x[c], c);
+                    if (!CHOL) {
+                        if (lane == 0 && piv == 0.0)
+                            report_info(u, p0 + c);
+                        const double l = lane > c ? x[c] * (1.0 / piv) : 0.0;
+                        if (lane > c)
+                            x[c] = l;
+                        // lane c's own row keeps its values (multiplier 0)
+#pragma unroll
+                        for (int k = c + 1; k < 16; k++)
+                            x[k] = fma(-l, readlane_d(x[k], c), x[k]);
+                    } else {
+                        if (lane == 0 && !(piv > 0.0))
+                            report_info(u, p0 + c);
+                        const double d = sqrt(piv);
+                        if (lane > c)
+                            x[c] *= 1.0 / d;
+                        else if (lane == c)
+                            x[c] = d;
+                        const double l = lane > c ? x[c] : 0.0;
+#pragma unroll
+                        for (int k = c + 1; k < 16; k++)  // L[p0 + k][c] is lane k's
+                            x[k] = fma(-l, readlane_d(x[c], k), x[k]);
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 16; c++)
+                    if (in)
+                        P[r * BF_LD + c] = x[c];
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            // sweep 2: rows p0 + 64 .. 127 against the finished diagonal
+            // block in LDS (LU: L21 = A21 inv(U11); Cholesky: A21 inv(L11^T))
+            if (p0 + 64 < SMALL_MAX) {
+                const int r = p0 + 64 + lane;
+                const bool in = r < SMALL_MAX;
+                double y[16];
+#pragma unroll
+                for (int c = 0; c < 16; c++)
+                    y[c] = in ? P[r * BF_LD + c] : 0.0;
+#pragma unroll
+                for (int c = 0; c < 16; c++) {
+                    const double *dr = P + (p0 + c) * BF_LD;  // LU: row c of U11
+                    if (!CHOL) {
+                        y[c] *= 1.0 / dr[c];
+#pragma unroll
+                        for (int k = c + 1; k < 16; k++)
+                            y[k] = fma(-y[c], dr[k], y[k]);
+                    } else {
+                        y[c] *= 1.0 / dr[c];
+#pragma unroll
+                        for (int k = c + 1; k < 16; k++)  // L11[k][c]
+                            y[k] = fma(-y[c], P[(p0 + k) * BF_LD + c], y[k]);
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 16; c++)
+                    if (in)
+                        P[r * BF_LD + c] = y[c];
+                asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            }
+            if (!CHOL && lane < 16) {
+                // column `lane` of inv(L11) (unit lower): forward substitution
+                double z[16];
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    double s = r == lane ? 1.0 : 0.0;
+#pragma unroll
+                    for (int k = 0; k < r; k++)
+                        s = fma(-P[(p0 + r) * BF_LD + k], z[k], s);
+                    z[r] = s;
+                }
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    Li[r * BF_LD + lane] = z[r];
+            }
+            // the factored panel back into the owner's registers
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+                if (i >= p) {
+                    v4d blk;
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        blk[e] = P[(16 * i + 4 * e + q) * BF_LD + cl];
+                    if (CHOL && i == p)  // keep the diagonal block's upper part as loaded
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+                            if (4 * e + q < cl)
+                                blk[e] = 0.0;
+                    if (JJ > 1 && jp)
+                        R[JJ - 1][i] = blk;
+                    else
+                        R[0][i] = blk;
+                }
+        }
+        lds_barrier();
+        // ---- trailing update of this wave's block-columns j > p ----
+#pragma unroll
+        for (int jj = 0; jj < JJ; jj++) {
+            const int j = w + NW * jj;
+            if (j <= p || j >= nb)
+                continue;
+            if (!CHOL) {
+                // U12 = inv(L11) A12, block (p, j)
+                v4d up = R[jj][0];
+#pragma unroll
+                for (int i = 1; i < 8; i++)
+                    if (i == p)
+                        up = R[jj][i];
+                v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++)
+                    acc = mfma_f64(Li[cl * BF_LD + 4 * kk + q], up[kk], acc);
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (i == p)
+                        R[jj][i] = acc;
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (i > p && i < nb) {
+                        v4d c4 = R[jj][i];
+#pragma unroll
+                        for (int kk = 0; kk < 4; kk++)
+                            c4 = mfma_f64(-P[(16 * i + cl) * BF_LD + 4 * kk + q], acc[kk], c4);
+                        R[jj][i] = c4;
+                    }
+            } else {
+                double bj[4];
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++)
+                    bj[kk] = P[(16 * j + cl) * BF_LD + 4 * kk + q];
+#pragma unroll
+                for (int i = 0; i < 8; i++)
+                    if (i >= j && i < nb) {
+                        v4d c4 = R[jj][i];
+#pragma unroll
+                        for (int kk = 0; kk < 4; kk++)
+                            c4 = mfma_f64(-P[(16 * i + cl) * BF_LD + 4 * kk + q], bj[kk], c4);
+                        R[jj][i] = c4;
+                    }
+            }
+        }
+        lds_barrier();
+    }
+
+#pragma unroll
+    for (int jj = 0; jj < JJ; jj++)
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = 16 * i + 4 * e + q, c = 16 * (w + NW * jj) + cl;
+                if (r < n && c < n && (!CHOL || r >= c))
+                    a[r + (size_t)c * lda] = R[jj][i][e];
+            }
+    __builtin_amdgcn_s_setprio(0);
+}
+
 // LDS (doubles) one workgroup needs for any item.
 constexpr int LDS_GEMM = 2 * (LDS_I > LDS_K ? LDS_I : LDS_K) * 2;  // 2 operands x 2 buffers
 constexpr int LDS_TRSM = TRSM_NMAX * 16 + 16 * 17;
-constexpr int LDS_SMALL = 516;
+constexpr int LDS_SMALL = LDS_SMALL_BLK;
 static_assert(TRINV_BLOCKS * 16 * 17 <= LDS_TRSM, "TRINV staging fits the generic LDS");
 static_assert(TRINV_BLOCKS * 16 == WG_THREADS, "TRINV: one thread per block column");
 static_assert(4 * GEMM_BM * 20 * 2 <= LDS_GEMM * 8 && 4 * GEMM_BM * 17 * 4 <= LDS_GEMM * 8,
@@ -1174,7 +1424,22 @@ __device__ __attribute__((always_inline)) inline void run_gemm_narrow_item(const
         gemm_narrow<false>(u, it, smem);
 }
 
-// GEMM/SYRK item of a unit without F_NARROW.
+// One GEMM output block of a unit without F_NARROW, any transposition.
+__device__ __attribute__((always_inline)) inline void gemm_any_block(const UnitDesc &u, int bi,
+                                                                     int bj, double *smem)
+{
+    switch (u.flags & (F_TA | F_TB)) {
+    case 0: gemm_dispatch_vec<false, false>(u, bi, bj, false, smem); break;
+    case F_TB: gemm_dispatch_vec<false, true>(u, bi, bj, false, smem); break;
+    case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
+    default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
+    }
+}
+
+// GEMM/SYRK item of a unit without F_NARROW.  GROUPED: the unit's items
+// cover grp x grp output blocks (the persistent kernel's lowering); the
+// level kernels run one block per item and keep the single-block form.
+template <bool GROUPED = false>
 __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDesc &u, int it,
                                                                     double *smem)
 {
@@ -1184,10 +1449,13 @@ __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDe
         gemm_dispatch_vec<false, true>(u, bi, bj, bi == bj, smem);
         return;
     }
-    // an item covers g x g output blocks, computed one after the other (a
-    // barrier between them: the next block's staging overwrites the LDS
-    // the previous one read last)
-    const int g = u.grp > 1 ? u.grp : 1;
+    if (!GROUPED || u.grp <= 1) {
+        gemm_any_block(u, it % u.gm, it / u.gm, smem);
+        return;
+    }
+    // g x g output blocks one after the other, a barrier between them (the
+    // next block's staging overwrites the LDS the previous one read last)
+    const int g = u.grp;
     const int gm = (u.gm + g - 1) / g;
     const int bi0 = (it % gm) * g, bj0 = (it / gm) * g;
 #pragma unroll 1
@@ -1197,12 +1465,7 @@ __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDe
             continue;
         if (q)
             lds_barrier();
-        switch (u.flags & (F_TA | F_TB)) {
-        case 0: gemm_dispatch_vec<false, false>(u, bi, bj, false, smem); break;
-        case F_TB: gemm_dispatch_vec<false, true>(u, bi, bj, false, smem); break;
-        case F_TA: gemm_dispatch_vec<true, false>(u, bi, bj, false, smem); break;
-        default: gemm_dispatch_vec<true, true>(u, bi, bj, false, smem); break;
-        }
+        gemm_any_block(u, bi, bj, smem);
     }
 }
 
@@ -1286,15 +1549,17 @@ __device__ __attribute__((always_inline)) inline void run_trsm_small_item(const 
 }
 constexpr int LDS_TRSM_SMALL = SMALL_MAX * 16 + 16 * 17;
 
-// Small GETRF/POTRF or CVT item (the only kinds k_batch_small runs).
+// Small GETRF/POTRF or CVT item (the only kinds k_batch_small runs), on NW
+// waves; a CVT item uses the first four.
+template <int NW = 4>
 __device__ __attribute__((always_inline)) inline void run_small_item(const UnitDesc &u, int it,
                                                                      double *smem)
 {
     if (u.op == OP_GETRF)
-        getrf_small(u, smem);
+        factor_small_blk<false, NW>(u, smem);
     else if (u.op == OP_POTRF)
-        potrf_small(u, smem);
-    else
+        factor_small_blk<true, NW>(u, smem);
+    else if (threadIdx.x < WG_THREADS)
         cvt_block(u, it);
 }
 
@@ -1330,8 +1595,8 @@ __device__ __attribute__((always_inline)) inline void run_item(const UnitDesc &u
         default: break;
         }
         break;
-    case OP_GETRF: getrf_small(u, smem); break;
-    case OP_POTRF: potrf_small(u, smem); break;
+    case OP_GETRF: factor_small_blk<false>(u, smem); break;
+    case OP_POTRF: factor_small_blk<true>(u, smem); break;
     case OP_CVT: cvt_block(u, it); break;
     case OP_TRINV: run_trinv_item(u, it, smem); break;
     default: break;

@@ -84,32 +84,69 @@ constexpr int LDS_PERSIST_B = LDS_GEMM > LDS_TRSM_SMALL ? LDS_GEMM : LDS_TRSM_SM
 constexpr int LDS_PERSIST = LDS_PERSIST_A > LDS_PERSIST_B ? LDS_PERSIST_A : LDS_PERSIST_B;
 static_assert(LDS_PERSIST >= LDS_SMALL, "persistent LDS holds the small factorizations");
 
-__device__ __attribute__((always_inline)) inline void run_persist_item(const UnitDesc &d, int it,
-                                                                       double *smem)
+// The kernel's LDS, at file scope so the item bodies below address it as LDS
+// without a pointer argument.
+__shared__ double p_smem[LDS_PERSIST];
+
+// One function per item kind, each kept out of line: its registers are
+// allocated for that body alone, and nothing of it is hoisted into the
+// worker loop (inlined into the loop, the loop-invariant parts of every
+// body stayed live across items and the GEMM loop spilled).
+__device__ __noinline__ void persist_gemm(const UnitDesc *ud, int it)
 {
-    switch (d.op) {
+    const UnitDesc d = *ud;
+    run_gemm_item<true>(d, it, p_smem);
+}
+__device__ __noinline__ void persist_gemm_narrow(const UnitDesc *ud, int it)
+{
+    const UnitDesc d = *ud;
+    run_gemm_narrow_item(d, it, p_smem);
+}
+__device__ __noinline__ void persist_trsm_inv(const UnitDesc *ud, int it)
+{
+    const UnitDesc d = *ud;
+    run_trsm_inv_item<4, PERSIST_TRSM_MAX>(d, it, p_smem);
+}
+__device__ __noinline__ void persist_trsm_small(const UnitDesc *ud, int it)
+{
+    const UnitDesc d = *ud;
+    run_trsm_small_item(d, it, p_smem);
+}
+__device__ __noinline__ void persist_trinv(const UnitDesc *ud, int it)
+{
+    const UnitDesc d = *ud;
+    run_trinv_item(d, it, p_smem);
+}
+__device__ __noinline__ void persist_small(const UnitDesc *ud, int it)
+{
+    const UnitDesc d = *ud;
+    run_small_item(d, it, p_smem);  // GETRF, POTRF, CVT
+}
+
+__device__ __forceinline__ void run_persist_item(const UnitDesc *ud, int op, int flags, int it)
+{
+    switch (op) {
     case OP_GEMM:
-        if (d.flags & F_NARROW)
-            run_gemm_narrow_item(d, it, smem);
+        if (flags & F_NARROW)
+            persist_gemm_narrow(ud, it);
         else
-            run_gemm_item(d, it, smem);
+            persist_gemm(ud, it);
         break;
-    case OP_SYRK: run_gemm_item(d, it, smem); break;
+    case OP_SYRK: persist_gemm(ud, it); break;
     case OP_TRSM:
-        if (d.flags & F_INV)
-            run_trsm_inv_item<4, PERSIST_TRSM_MAX>(d, it, smem);
+        if (flags & F_INV)
+            persist_trsm_inv(ud, it);
         else
-            run_trsm_small_item(d, it, smem);
+            persist_trsm_small(ud, it);
         break;
-    case OP_TRINV: run_trinv_item(d, it, smem); break;
-    default: run_small_item(d, it, smem); break;  // GETRF, POTRF, CVT
+    case OP_TRINV: persist_trinv(ud, it); break;
+    default: persist_small(ud, it); break;
     }
 }
 
 // The per-wave context of sched_worker on the device.
 struct DevWave {
     const ExecView &ev;
-    double *smem;
     int *s_item;
     int *s_last;
     __device__ __forceinline__ int wave() const
@@ -141,11 +178,11 @@ struct DevWave {
     }
     __device__ __forceinline__ void run(uint32_t u, int item) const
     {
-        // a private copy of the descriptor lives in registers
-        const UnitDesc d = ev.units[u];
+        const UnitDesc *ud = ev.units + u;
+        const int op = ud->op, flags = ud->flags, base = (int)ud->item_base;
         if (ev.trace && threadIdx.x == 0)
             ev.trace[2 * (size_t)item] = wall_clock64();
-        run_persist_item(d, item - (int)d.item_base, smem);
+        run_persist_item(ud, op, flags, item - base);
         if (ev.trace && threadIdx.x == 0)
             ev.trace[2 * (size_t)item + 1] = wall_clock64();
     }
@@ -157,9 +194,8 @@ struct DevWave {
 __global__ __launch_bounds__(WG_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 k_persistent(ExecView ev)
 {
-    __shared__ double smem[LDS_PERSIST];
     __shared__ int s_item, s_last;
-    DevWave w{ev, smem, &s_item, &s_last};
+    DevWave w{ev, &s_item, &s_last};
     sched_worker<DevAtom>(ev, w);
 }
 
