@@ -7,7 +7,10 @@
 //   * mustard::TiledGraphCreator with subgraphs, graph, subgraphDependencies,
 //     beginCaptureOperation, endCaptureOperation, printDeps,
 //     insertDependencyKernel, insertSubgraph (mustard/mustard.h:132-248);
-//   * BrokerWorkDistributor(size) / free_mem() (mustard/broker_queue.h:26-43).
+//   * BrokerWorkDistributor(size), free_mem() and the device enqueue /
+//     dequeue / size (mustard/broker_queue.h:12-150), in mustard_queue.h.
+//     The executables' persistent scheduler keeps its own per-priority rings
+//     in HBM; the class is for kernels written against the reference's queue.
 //
 // What differs, and why:
 //   * Between beginCaptureOperation and endCaptureOperation the caller issues
@@ -19,8 +22,12 @@
 //     instantiate() lowers the recorded DAG either to one hipGraph (the
 //     reference's --tiled mode) or to one persistent scheduler kernel with
 //     its queue and dependency counters in HBM (--subgraph mode).
-//   * Large diagonal factorizations are nested into sub-tiles automatically
-//     (the reference does it by hand with insertSubgraph for B > MAX_TILE).
+//   * Large diagonal factorizations are nested into sub-tiles automatically;
+//     insertSubgraph still nests a hand-recorded sub-graph the way the
+//     reference does for B > MAX_TILE (lu_mustard.cu:381-388): a creator
+//     built on a graph handle that ends without being instantiated leaves its
+//     recorded DAG under that handle, and insertSubgraph(handle) makes it a
+//     nested sub-DAG of the last ended task.
 //   * Errors print a message and exit(EXIT_FAILURE), as checkCudaErrors does
 //     (include/utils.h:8-18).
 #pragma once
@@ -32,11 +39,13 @@
 #include <fstream>
 #include <initializer_list>
 #include <iostream>
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "mustard_hip.h"
+#include "mustard_queue.h"
 
 #define FLAGS_SUBG_COUNT 0
 #define FLAGS_OCCUP 4
@@ -77,17 +86,6 @@ enum class Mode : int {
     Stream = MU_MODE_STREAM,     // units launched one by one (debugging)
 };
 
-// The reference's queue type (mustard/broker_queue.h).  The queue rings now
-// live inside each executable; this handle keeps the reference's
-// construction and free_mem() calls valid and records the requested size.
-class BrokerWorkDistributor {
-  public:
-    int N;
-    explicit BrokerWorkDistributor(int size) : N(size) {}
-    void free_mem() {}
-    int size() const { return N; }
-};
-
 // A lowered, uploaded DAG (the instantiated graph / scheduler of the
 // reference, lu_mustard.cu:578-600, 636).
 class Executable {
@@ -122,6 +120,26 @@ class Executable {
     mu_exec_t h_ = nullptr;
 };
 
+// DAGs recorded by creators built on a graph handle that ended without being
+// instantiated (the reference's recordSubgraph leaves its tasks in the
+// cudaGraph_t it returns, lu_mustard.cu:133-254); insertSubgraph reads them.
+inline std::map<hipGraph_t, mu_dag_t> &recordedGraphs()
+{
+    static std::map<hipGraph_t, mu_dag_t> m;
+    return m;
+}
+
+// Drop the DAG recorded under `graph` (after its last insertSubgraph).
+inline void releaseGraph(hipGraph_t graph)
+{
+    auto &m = recordedGraphs();
+    auto it = m.find(graph);
+    if (it != m.end()) {
+        mu_dag_destroy(it->second);
+        m.erase(it);
+    }
+}
+
 class TiledGraphCreator {
   public:
     // Per-task graphs do not exist on HIP (no device-side graph launch); the
@@ -143,7 +161,13 @@ class TiledGraphCreator {
     {
         if (capturing_)
             mu_dag_capture_end(dag_, stream_);
-        mu_dag_destroy(dag_);
+        capturing_ = false;
+        if (!instantiated_ && graph && numTasks() > 0) {
+            releaseGraph(graph);
+            recordedGraphs()[graph] = dag_;  // kept for insertSubgraph(graph)
+        } else {
+            mu_dag_destroy(dag_);
+        }
     }
     TiledGraphCreator(const TiledGraphCreator &) = delete;
     TiledGraphCreator &operator=(const TiledGraphCreator &) = delete;
@@ -196,14 +220,27 @@ class TiledGraphCreator {
         checkMustard(mu_dag_add_edge(dag_, src, dst));
     }
 
-    // mustard/mustard.h:214-248.  Nesting of large diagonal factorizations is
-    // done by the library when the DAG is lowered, for every tile larger than
-    // the LDS-resident sub-tile, so a hand-built child graph is never needed.
-    void insertSubgraph(hipGraph_t)
+    // mustard/mustard.h:214-248: the sub-graph recorded on `getrfSubgraph`
+    // (by a creator built on that handle and since destroyed) runs as a
+    // nested sub-DAG of the last ended task, after that task's own
+    // operations.  The library also nests large diagonal factorizations by
+    // itself, so recording mu_dgetrf_nopiv on the whole tile is equivalent.
+    void insertSubgraph(hipGraph_t getrfSubgraph)
     {
-        std::fprintf(stderr, "insertSubgraph: nested tile factorizations are lowered by the "
-                             "library; record mu_dgetrf_nopiv on the whole tile instead\n");
-        std::exit(EXIT_FAILURE);
+        auto it = recordedGraphs().find(getrfSubgraph);
+        if (it == recordedGraphs().end()) {
+            std::fprintf(stderr, "insertSubgraph: no sub-graph was recorded on this graph "
+                                 "handle\n");
+            std::exit(EXIT_FAILURE);
+        }
+        checkMustard(mu_dag_insert_subdag(dag_, it->second));
+    }
+
+    // The same with a creator still in scope (its recording ends here).
+    void insertSubgraph(TiledGraphCreator &child)
+    {
+        child.endRecording();
+        checkMustard(mu_dag_insert_subdag(dag_, child.dag_));
     }
 
     // ---- additions: lowering and execution --------------------------------
@@ -224,13 +261,21 @@ class TiledGraphCreator {
         return n;
     }
 
-    // End recording and lower the DAG for `mode`.
-    Executable instantiate(Mode mode, const mu_exec_opts_t *opts = nullptr)
+    // Stop recording the stream's tile operations into this DAG (the stream
+    // returns to an enclosing recording, if any, or to running them).
+    void endRecording()
     {
         if (capturing_) {
             checkMustard(mu_dag_capture_end(dag_, stream_));
             capturing_ = false;
         }
+    }
+
+    // End recording and lower the DAG for `mode`.
+    Executable instantiate(Mode mode, const mu_exec_opts_t *opts = nullptr)
+    {
+        endRecording();
+        instantiated_ = true;
         mu_exec_t h = nullptr;
         checkMustard(mu_exec_create(dag_, (int)mode, opts, &h));
         return Executable(h);
@@ -263,6 +308,7 @@ class TiledGraphCreator {
     hipStream_t stream_;
     bool subgraph_;
     bool capturing_ = false;
+    bool instantiated_ = false;
     int current_ = -1;
     MatrixTile lastModifiedTile_ = std::make_pair(-1, -1);
 };

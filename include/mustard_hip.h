@@ -146,6 +146,16 @@ int mu_dag_num_tasks(mu_dag_t dag, int *ntasks);
 int mu_dag_task_deps(mu_dag_t dag, int task, int *deps, int max, int *ndeps);
 /* insertDependencyKernel(src, dst, ...) -- mustard.h:197-211 */
 int mu_dag_add_edge(mu_dag_t dag, int src, int dst);
+/* insertSubgraph (mustard.h:214-248, used at lu_mustard.cu:381-388): the
+ * tasks of `child` (recorded like any DAG, typically the tiled factorization
+ * of one large diagonal tile, lu_mustard.cu:133-254) become a nested
+ * sub-DAG of the last task ended in `dag`; it runs after that task's own
+ * operations and before the task counts as done, with the child's own
+ * dependencies inside it.  The child is copied: it may be destroyed (or
+ * inserted again) afterwards; device buffers it owns stay alive with `dag`.
+ * Recordings on one stream nest: capture_begin of the child on the parent's
+ * stream records there until the child's capture_end. */
+int mu_dag_insert_subdag(mu_dag_t dag, mu_dag_t child);
 /* Problem tag for batched recording: tiles of different tags never alias. */
 int mu_dag_set_tag(mu_dag_t dag, int tag);
 /* Operation codes reported by mu_dag_task_info. */
@@ -166,7 +176,9 @@ int mu_dag_task_info(mu_dag_t dag, int task, int *w_row, int *w_col, int *ops, i
  * mu_exec_create lowers the DAG and uploads it (the reference's graph
  * instantiation, lu_mustard.cu:543-604 / 633-636); mu_exec_launch replays it
  * (cudaGraphLaunch of the scheduler or of the tiled graph, lu_mustard.cu:609,
- * 640).  The recorded device pointers must stay valid. */
+ * 640).  The caller's recorded device pointers must stay valid; buffers the
+ * library allocated for the DAG (reduced-precision panel copies) live until
+ * the DAG and every executable built from it are destroyed, in any order. */
 int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t *exec);
 int mu_exec_launch(mu_exec_t exec, void *stream);
 /* Tiled executables: launch the batched kernel of one graph node alone, and
@@ -247,15 +259,17 @@ int mu_record_batch(mu_dag_t dag, int nprob, const int *kinds, const int *ns, do
  * solves, residual and correction in FP64), followed by the refinement of
  * LAPACK dsgesv/dsposv: x = x + solve(b - A x) until
  * ||r_j||_inf <= ||x_j||_inf ||A||_inf 2^-53 sqrt(n) for every column, at
- * most itermax times (0: 30); with `fallback`, a run that does not converge
- * is refactored and solved in FP64.  The factors live in a library-owned
+ * most itermax times (0: 30); with `fallback`, a run whose reduced-precision
+ * factorization breaks down (info > 0; iters is then 0) or whose refinement
+ * does not converge is refactored and solved in FP64.  The factors live in a library-owned
  * n x n workspace.  Graph construction happens before the timed region. */
 typedef struct {
     int kind;     /* MU_SOLVE_LU or MU_SOLVE_CHOL */
     int tiles;    /* tiles per dimension (n % tiles == 0) */
     int prec;     /* MU_PREC_FP64 / MU_PREC_FP32 / MU_PREC_BF16 trailing updates */
     int itermax;  /* refinement iterations cap; 0 = 30 */
-    int fallback; /* 1: refactor in FP64 when refinement does not converge */
+    int fallback; /* 1: refactor in FP64 when the reduced-precision factorization
+                     fails or the refinement does not converge */
 } mu_ir_opts_t;
 typedef struct {
     int info;            /* factorization info (0 = fine) */

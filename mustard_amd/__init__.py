@@ -170,6 +170,11 @@ class Dag:
     def add_edge(self, src, dst):
         call("mu_dag_add_edge", self.h, src, dst)
 
+    def insert_subdag(self, child):
+        """insertSubgraph (mustard.h:214-248): child's tasks run as a nested
+        sub-DAG of the last task ended here, after its own operations."""
+        call("mu_dag_insert_subdag", self.h, child.h)
+
 
 class Exec:
     """A lowered, uploaded DAG ready to launch."""

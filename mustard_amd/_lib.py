@@ -98,6 +98,7 @@ SIGNATURES = {
     "mu_dag_num_tasks": (_i, [_vp, _ip]),
     "mu_dag_task_deps": (_i, [_vp, _i, _ip, _i, _ip]),
     "mu_dag_add_edge": (_i, [_vp, _i, _i]),
+    "mu_dag_insert_subdag": (_i, [_vp, _vp]),
     "mu_dag_set_tag": (_i, [_vp, _i]),
     "mu_dag_task_info": (_i, [_vp, _i, _ip, _ip, _ip, _i, _ip]),
     "mu_exec_create": (_i, [_vp, _i, ctypes.POINTER(ExecOpts), ctypes.POINTER(_vp)]),

@@ -10,6 +10,59 @@ int myPE = 0;
 
 static cli::MustardConfig cfg;
 
+// lu_mustard.cu:133-254: the tiled LU of one diagonal tile (order B, leading
+// dimension N) in subT x subT sub-tiles, recorded as its own graph for
+// insertSubgraph.  The creator is built on a fresh graph handle and ends
+// without being instantiated, so its tasks stay under that handle.
+static hipGraph_t recordSubgraph(double *subMatrix, int subT, hipStream_t s, int *d_info)
+{
+    const int N = (int)cfg.N, subN = (int)cfg.B, subB = subN / subT;
+    std::cout << "PE " << myPE << ". Create subgraph with subN=" << subT << " (" << subT
+              << " of " << subB << "x" << subB << " tiles)" << std::endl;
+    auto getMatrixBlock = [&](double *matrix, int i, int j) {
+        return matrix + (size_t)i * subB + (size_t)j * subB * N;
+    };
+    int totalNodes = subT;
+    for (int k = 0; k < subT; k++)
+        for (int i = k + 1; i < subT; i++)
+            totalNodes += 2 + (subT - (k + 1));
+    hipGraph_t subGraph;
+    checkHip(hipGraphCreate(&subGraph, 0));
+    mustard::TiledGraphCreator creator(s, subGraph, false, totalNodes);
+    for (int k = 0; k < subT; k++) {
+        creator.beginCaptureOperation(std::make_pair(k, k), {std::make_pair(k, k)});
+        checkMustard(mu_dgetrf_nopiv(s, subB, getMatrixBlock(subMatrix, k, k), N, d_info));
+        creator.endCaptureOperation();
+        for (int i = k + 1; i < subT; i++) {
+            creator.beginCaptureOperation(std::make_pair(k, i),
+                                          {std::make_pair(k, k), std::make_pair(k, i)});
+            checkMustard(mu_dtrsm(s, 'L', 'L', 'N', 'U', subB, subB, 1.0,
+                                  getMatrixBlock(subMatrix, k, k), N,
+                                  getMatrixBlock(subMatrix, k, i), N));
+            creator.endCaptureOperation();
+        }
+        for (int i = k + 1; i < subT; i++) {
+            creator.beginCaptureOperation(std::make_pair(i, k),
+                                          {std::make_pair(k, k), std::make_pair(i, k)});
+            checkMustard(mu_dtrsm(s, 'R', 'U', 'N', 'N', subB, subB, 1.0,
+                                  getMatrixBlock(subMatrix, k, k), N,
+                                  getMatrixBlock(subMatrix, i, k), N));
+            creator.endCaptureOperation();
+            for (int j = k + 1; j < subT; j++) {
+                creator.beginCaptureOperation(
+                    std::make_pair(i, j),
+                    {std::make_pair(i, k), std::make_pair(k, j), std::make_pair(i, j)});
+                checkMustard(mu_dgemm(s, 'N', 'N', subB, subB, subB, -1.0,
+                                      getMatrixBlock(subMatrix, i, k), N,
+                                      getMatrixBlock(subMatrix, k, j), N, 1.0,
+                                      getMatrixBlock(subMatrix, i, j), N));
+                creator.endCaptureOperation();
+            }
+        }
+    }
+    return subGraph;
+}
+
 static void tiledLU(const cli::Args &cmdl, bool subgraph)
 {
     const size_t N = cfg.N, B = cfg.B, T = cfg.T;
@@ -44,8 +97,21 @@ static void tiledLU(const cli::Args &cmdl, bool subgraph)
     for (size_t k = 0; k < T; k++) {
         // A[k][k] = GETRF(A[k][k])
         creator.beginCaptureOperation(std::make_pair((int)k, (int)k), {std::make_pair((int)k, (int)k)});
-        checkMustard(mu_dgetrf_nopiv(s, b, getMatrixBlock(d_matrix, k, k), n, d_info));
+        // lu_mustard.cu:358-388: with --subgraph a tile above MAX_TILE is
+        // factored by a nested sub-graph of ceil(B / MAX_TILE)^2 sub-tiles
+        // (when they divide the tile evenly; otherwise the library's own
+        // nesting of the whole-tile GETRF does the same job)
+        const int subT = (int)((B + MAX_TILE - 1) / MAX_TILE);
+        const bool nested = subgraph && B > MAX_TILE && B % subT == 0;
+        if (!nested)
+            checkMustard(mu_dgetrf_nopiv(s, b, getMatrixBlock(d_matrix, k, k), n, d_info));
         creator.endCaptureOperation();
+        if (nested) {
+            hipGraph_t subLU = recordSubgraph(getMatrixBlock(d_matrix, k, k), subT, s, d_info);
+            creator.insertSubgraph(subLU);
+            mustard::releaseGraph(subLU);
+            checkHip(hipGraphDestroy(subLU));
+        }
         for (size_t i = k + 1; i < T; i++) {
             // U[k][i] = L[k][k]^-1 A[k][i]
             creator.beginCaptureOperation(std::make_pair((int)k, (int)i),

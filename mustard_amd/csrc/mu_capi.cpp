@@ -1,6 +1,7 @@
 // mu_capi.cpp -- the extern "C" entry points of libmustard_hip.so.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -14,22 +15,35 @@
 #include <vector>
 
 #include "../../include/mustard_hip.h"
+#include "mu_capi.hpp"
 #include "mu_dag.hpp"
 #include "mu_runtime.hpp"
 #include "mu_sched_host.hpp"
 
 
-struct mu_dag_s {
-    mu::Dag dag;
-    std::vector<void *> owned;  // device buffers the recorded ops use (freed with the DAG)
+// Device buffers the recorded operations of a DAG use (the reduced-precision
+// panel shadows): owned jointly by the DAG and every executable built from
+// it, and freed when the last of them is destroyed.
+struct OwnedBuffers {
+    std::vector<void *> p;
+    // buffers of sub-graphs inserted into this DAG (mu_dag_insert_subdag)
+    std::vector<std::shared_ptr<OwnedBuffers>> sub;
+    ~OwnedBuffers()
+    {
+        for (void *q : p)
+            (void)hipFree(q);
+    }
 };
 
-namespace mu {
-int set_error(int code, const std::string &msg);
-}
+struct mu_dag_s {
+    mu::Dag dag;
+    std::shared_ptr<OwnedBuffers> owned = std::make_shared<OwnedBuffers>();
+};
+
 
 struct mu_exec_s {
     mu::Exec *e = nullptr;
+    std::shared_ptr<OwnedBuffers> owned;  // the DAG's buffers its units point into
 };
 
 namespace {
@@ -56,18 +70,31 @@ namespace {
     } while (0)
 
 // Streams being recorded into a DAG, and the update precision per stream.
+// Recordings nest: a sub-graph recorded on the stream of a graph that is
+// being recorded (the reference's recordSubgraph, lu_mustard.cu:133-254)
+// takes the stream until its recording ends, then the outer one resumes.
+struct StreamRec {
+    std::vector<mu_dag_t> dags;  // innermost last
+    int prec = MU_PREC_FP64;
+};
 struct StreamState {
-    mu_dag_t dag = nullptr;
+    mu_dag_t dag = nullptr;  // the innermost recording DAG
     int prec = MU_PREC_FP64;
 };
 std::mutex g_mu;
-std::map<void *, StreamState> g_streams;
+std::map<void *, StreamRec> g_streams;
 
 StreamState stream_state(void *s)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_streams.find(s);
-    return it == g_streams.end() ? StreamState() : it->second;
+    StreamState st;
+    if (it != g_streams.end()) {
+        st.prec = it->second.prec;
+        if (!it->second.dags.empty())
+            st.dag = it->second.dags.back();
+    }
+    return st;
 }
 
 int prec_flags(int prec)
@@ -102,6 +129,56 @@ bool is_n(char c) { return c == 'N' || c == 'n'; }
 char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 'a' + 'A') : c; }
 
 }  // namespace
+
+// mu_dtrsm's arguments as one recorded operation (validated).
+int mu::dtrsm_op(char side, char uplo, char transa, char diag, int m, int n, double alpha,
+                 const double *A, int lda, double *B, int ldb, OpRec &op)
+{
+    side = up(side);
+    uplo = up(uplo);
+    diag = up(diag);
+    const bool t = is_t(transa);
+    if ((side != 'L' && side != 'R') || (uplo != 'L' && uplo != 'U') ||
+        (!t && !is_n(transa)) || (diag != 'U' && diag != 'N'))
+        return fail(MU_EINVAL, "mu_dtrsm: bad side/uplo/trans/diag");
+    if (m < 0 || n < 0 || ldb < std::max(1, m) || lda < std::max(1, side == 'L' ? m : n))
+        return fail(MU_EINVAL, "mu_dtrsm: bad size or leading dimension");
+    op = OpRec();
+    if (alpha == 0.0) {
+        // BLAS: alpha = 0 sets B to zero and never reads A; recorded as a
+        // GEMM with k = 0 and beta = 0 (C = 0 without reading C)
+        op.op = OP_GEMM;
+        op.flags = F_TB;  // the 64 x 64 block path, which skips every operand at k = 0
+        op.m = m;
+        op.n = n;
+        op.k = 0;
+        op.alpha = 0.0;
+        op.beta = 0.0;
+        op.lda = op.ldb = 1;
+        op.c = B;
+        op.ldc = ldb;
+        return MU_OK;
+    }
+    // Reduce to T Y = alpha Z with T lower triangular:
+    //   left:  op(A) X = B      -> T = op(A),    Z = B
+    //   right: X op(A) = B      -> T = op(A)^T,  Z = B^T
+    // An upper T is handled by reversing the row order (F_REV).
+    const bool tt = side == 'L' ? t : !t;  // T = A^T
+    const bool t_upper = tt ? (uplo == 'L') : (uplo == 'U');
+    op.op = OP_TRSM;
+    op.flags = (tt ? F_T_TRANS : 0) | (t_upper ? F_REV : 0) | (side == 'R' ? F_Z_ROWS : 0) |
+               (diag == 'U' ? F_UNIT : 0);
+    op.m = side == 'L' ? m : n;  // order of the triangle
+    op.n = side == 'L' ? n : m;  // right-hand sides
+    op.alpha = alpha;
+    op.a = A;
+    op.lda = lda;
+    op.c = B;
+    op.ldc = ldb;
+    return MU_OK;
+}
+
+bool mu::stream_recording(void *stream) { return stream_state(stream).dag != nullptr; }
 
 extern "C" {
 
@@ -280,53 +357,9 @@ int mu_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha, co
 int mu_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
              double alpha, const double *A, int lda, double *B, int ldb)
 {
-    side = up(side);
-    uplo = up(uplo);
-    diag = up(diag);
-    const bool t = is_t(transa);
-    if ((side != 'L' && side != 'R') || (uplo != 'L' && uplo != 'U') ||
-        (!t && !is_n(transa)) || (diag != 'U' && diag != 'N'))
-        return fail(MU_EINVAL, "mu_dtrsm: bad side/uplo/trans/diag");
-    if (m < 0 || n < 0 || ldb < std::max(1, m) || lda < std::max(1, side == 'L' ? m : n))
-        return fail(MU_EINVAL, "mu_dtrsm: bad size or leading dimension");
-    if (alpha == 0.0) {
-        // BLAS: alpha = 0 sets B to zero and never reads A; recorded as a
-        // GEMM with k = 0 and beta = 0 (C = 0 without reading C)
-        mu::OpRec z;
-        z.op = mu::OP_GEMM;
-        z.flags = mu::F_TB;  // the 64 x 64 block path, which skips every operand at k = 0
-        z.m = m;
-        z.n = n;
-        z.k = 0;
-        z.alpha = 0.0;
-        z.beta = 0.0;
-        z.lda = z.ldb = 1;
-        z.c = B;
-        z.ldc = ldb;
-        return issue(stream, z);
-    }
-    // Reduce to T Y = alpha Z with T lower triangular:
-    //   left:  op(A) X = B      -> T = op(A),    Z = B
-    //   right: X op(A) = B      -> T = op(A)^T,  Z = B^T
-    // An upper T is handled by reversing the row order (F_REV).
-    bool tt;  // T = A^T
-    if (side == 'L')
-        tt = t;
-    else
-        tt = !t;
-    const bool t_upper = tt ? (uplo == 'L') : (uplo == 'U');
     mu::OpRec op;
-    op.op = mu::OP_TRSM;
-    op.flags = (tt ? mu::F_T_TRANS : 0) | (t_upper ? mu::F_REV : 0) |
-               (side == 'R' ? mu::F_Z_ROWS : 0) | (diag == 'U' ? mu::F_UNIT : 0);
-    op.m = side == 'L' ? m : n;  // order of the triangle
-    op.n = side == 'L' ? n : m;  // right-hand sides
-    op.alpha = alpha;
-    op.a = A;
-    op.lda = lda;
-    op.c = B;
-    op.ldc = ldb;
-    return issue(stream, op);
+    const int rc = mu::dtrsm_op(side, uplo, transa, diag, m, n, alpha, A, lda, B, ldb, op);
+    return rc != MU_OK ? rc : issue(stream, op);
 }
 
 int mu_dgetrf_nopiv(void *stream, int n, double *A, int lda, int *info_dev)
@@ -392,13 +425,12 @@ int mu_dag_destroy(mu_dag_t dag)
         return MU_OK;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        for (auto &kv : g_streams)
-            if (kv.second.dag == dag)
-                kv.second.dag = nullptr;
+        for (auto &kv : g_streams) {
+            auto &v = kv.second.dags;
+            v.erase(std::remove(v.begin(), v.end(), dag), v.end());
+        }
     }
-    for (void *p : dag->owned)
-        (void)hipFree(p);
-    delete dag;
+    delete dag;  // its buffers go with the last executable that still uses them
     return MU_OK;
 }
 
@@ -407,7 +439,10 @@ int mu_dag_capture_begin(mu_dag_t dag, void *stream)
     if (!dag)
         return fail(MU_EINVAL, "dag is NULL");
     std::lock_guard<std::mutex> lk(g_mu);
-    g_streams[stream].dag = dag;
+    auto &v = g_streams[stream].dags;
+    if (std::find(v.begin(), v.end(), dag) != v.end())
+        return fail(MU_ESTATE, "the DAG is already recording this stream");
+    v.push_back(dag);
     return MU_OK;
 }
 
@@ -415,9 +450,10 @@ int mu_dag_capture_end(mu_dag_t dag, void *stream)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_streams.find(stream);
-    if (it == g_streams.end() || it->second.dag != dag)
-        return fail(MU_ESTATE, "stream is not being recorded by this DAG");
-    it->second.dag = nullptr;
+    if (it == g_streams.end() || it->second.dags.empty() || it->second.dags.back() != dag)
+        return fail(MU_ESTATE, "stream is not being recorded by this DAG (or a sub-graph "
+                               "recording on it has not ended)");
+    it->second.dags.pop_back();
     return MU_OK;
 }
 
@@ -470,6 +506,16 @@ int mu_dag_add_edge(mu_dag_t dag, int src, int dst)
     if (!dag || src < 0 || dst <= src || dst >= (int)dag->dag.tasks.size())
         return fail(MU_EINVAL, "mu_dag_add_edge: edge must go from an earlier to a later task");
     dag->dag.add_edge(src, dst);
+    return MU_OK;
+}
+
+int mu_dag_insert_subdag(mu_dag_t dag, mu_dag_t child)
+{
+    if (!dag || !child || dag == child)
+        return fail(MU_EINVAL, "mu_dag_insert_subdag: bad arguments");
+    if (dag->dag.insert_sub(child->dag) != 0)
+        return fail(MU_ESTATE, dag->dag.err);
+    dag->owned->sub.push_back(child->owned);
     return MU_OK;
 }
 
@@ -539,6 +585,7 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
         return fail(MU_EHIP, err);
     *exec = new mu_exec_s();
     (*exec)->e = e;
+    (*exec)->owned = dag->owned;
     return MU_OK;
 }
 
@@ -757,7 +804,7 @@ int make_shadow(mu_dag_t dag, int n, int b, int prec, Shadow &sh)
     sh.b = b;
     void *p = nullptr;
     HIPTRY(hipMalloc(&p, (size_t)n * n * sh.elem));
-    dag->owned.push_back(p);
+    dag->owned->p.push_back(p);
     sh.p = (char *)p;
     return MU_OK;
 }

@@ -71,6 +71,37 @@ void Dag::add_edge(int src, int dst)
     }
 }
 
+int Dag::insert_sub(const Dag &child)
+{
+    if (cur_ >= 0) {
+        err = "insertSubgraph: an operation is being recorded";
+        return -1;
+    }
+    if (tasks.empty()) {
+        err = "insertSubgraph: no task recorded yet";
+        return -1;
+    }
+    if (child.cur_ >= 0) {
+        err = "insertSubgraph: the sub-graph is still recording an operation";
+        return -1;
+    }
+    std::vector<TaskRec> &sub = tasks.back().sub;
+    // a second insertion into the same task runs after the first one: its
+    // tasks follow the earlier ones, each also waiting for all of them
+    const int base = (int)sub.size();
+    for (const TaskRec &c : child.tasks) {
+        TaskRec t = c;
+        for (int &d : t.deps)
+            d += base;
+        for (int e = 0; e < base; e++)
+            t.deps.push_back(e);
+        std::sort(t.deps.begin(), t.deps.end());
+        t.deps.erase(std::unique(t.deps.begin(), t.deps.end()), t.deps.end());
+        sub.push_back(std::move(t));
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // lowering
 // ---------------------------------------------------------------------------
@@ -610,68 +641,105 @@ double item_cost_us(const UnitDesc &u)
 
 }  // namespace
 
-bool lower(const Dag &dag, const LowerOpts &opts, Lowered &out, std::string &err)
+// Lower one task: its operations in the order they were issued, then its
+// nested sub-DAG (if any).  `have` is false for a task without device work.
+static bool lower_task(Builder &b, const TaskRec &task, Frag &whole, bool &have, double &flops);
+
+// Lower a list of tasks whose deps index the same list; tf[i] is task i's
+// fragment (a task without device work passes on its predecessors' sinks).
+static bool lower_tasks(Builder &b, const std::vector<TaskRec> &tasks, std::vector<Frag> &tf,
+                        std::vector<int> *first_unit, std::vector<int> *nunits, double &flops,
+                        bool top)
 {
-    Builder b(opts);
-    std::vector<Frag> tf(dag.tasks.size());
-    out.task_first_unit.assign(dag.tasks.size(), -1);
-    out.task_nunits.assign(dag.tasks.size(), 0);
-    out.flops = 0.0;
-    for (size_t t = 0; t < dag.tasks.size(); t++) {
-        const TaskRec &task = dag.tasks[t];
-        b.task = (int)t;
+    tf.assign(tasks.size(), Frag());
+    for (size_t t = 0; t < tasks.size(); t++) {
+        const TaskRec &task = tasks[t];
+        if (top)
+            b.task = (int)t;
         const int first = (int)b.units.size();
         Frag whole;
         bool have = false;
-        for (const OpRec &op : task.ops) {
-            out.flops += op_flops(op);
-            Frag f;
-            if (!b.lower_op(op, f)) {
-                err = b.err;
+        if (!lower_task(b, task, whole, have, flops))
+            return false;
+        for (int p : task.deps)
+            if (p < 0 || p >= (int)t) {
+                b.err = "task dependency is not an earlier task";
                 return false;
             }
-            if (f.src.empty())
-                continue;
-            if (!have) {
-                whole = f;
-                have = true;
-            } else {
-                // ops of one task run in the order they were issued
-                for (int s : whole.snk)
-                    for (int d : f.src)
-                        b.edge(s, d);
-                whole.snk = f.snk;
-            }
-        }
         if (!have) {
-            // A task without device work passes its predecessors' completion
-            // straight on to its successors.
-            for (int p : task.deps) {
-                if (p < 0 || p >= (int)t) {
-                    err = "task dependency is not an earlier task";
-                    return false;
-                }
+            for (int p : task.deps)
                 whole.snk.insert(whole.snk.end(), tf[p].snk.begin(), tf[p].snk.end());
-            }
             std::sort(whole.snk.begin(), whole.snk.end());
             whole.snk.erase(std::unique(whole.snk.begin(), whole.snk.end()), whole.snk.end());
-            tf[t] = whole;
-            out.task_first_unit[t] = first;
-            out.task_nunits[t] = 0;
-            continue;
-        }
-        for (int p : task.deps) {
-            if (p < 0 || p >= (int)t) {
-                err = "task dependency is not an earlier task";
-                return false;
-            }
-            for (int s : tf[p].snk)
-                for (int d : whole.src)
-                    b.edge(s, d);
+        } else {
+            for (int p : task.deps)
+                for (int s : tf[p].snk)
+                    for (int d : whole.src)
+                        b.edge(s, d);
         }
         tf[t] = whole;
-        out.task_first_unit[t] = first;
-        out.task_nunits[t] = (int)b.units.size() - first;
+        if (first_unit) {
+            (*first_unit)[t] = first;
+            (*nunits)[t] = (int)b.units.size() - first;
+        }
+    }
+    return true;
+}
+
+static bool lower_task(Builder &b, const TaskRec &task, Frag &whole, bool &have, double &flops)
+{
+    have = false;
+    auto chain = [&](const Frag &f) {
+        if (!have) {
+            whole = f;
+            have = true;
+            return;
+        }
+        // ops of one task run in the order they were issued
+        for (int s : whole.snk)
+            for (int d : f.src)
+                b.edge(s, d);
+        whole.snk = f.snk;
+    };
+    for (const OpRec &op : task.ops) {
+        flops += op_flops(op);
+        Frag f;
+        if (!b.lower_op(op, f))
+            return false;
+        if (!f.src.empty())
+            chain(f);
+    }
+    if (!task.sub.empty()) {
+        // the sub-DAG's own sources and sinks: units of its range without a
+        // predecessor inside it, and without any successor (nothing later
+        // exists yet)
+        const int lo = (int)b.units.size();
+        std::vector<Frag> stf;
+        if (!lower_tasks(b, task.sub, stf, nullptr, nullptr, flops, false))
+            return false;
+        Frag f;
+        for (int u = lo; u < (int)b.units.size(); u++) {
+            if (!b.has_pred[u])
+                f.src.push_back(u);
+            if (b.succ[u].empty())
+                f.snk.push_back(u);
+        }
+        if (!f.src.empty())
+            chain(f);
+    }
+    return true;
+}
+
+bool lower(const Dag &dag, const LowerOpts &opts, Lowered &out, std::string &err)
+{
+    Builder b(opts);
+    std::vector<Frag> tf;
+    out.task_first_unit.assign(dag.tasks.size(), -1);
+    out.task_nunits.assign(dag.tasks.size(), 0);
+    out.flops = 0.0;
+    if (!lower_tasks(b, dag.tasks, tf, &out.task_first_unit, &out.task_nunits, out.flops, true)) {
+        err = b.err;
+        return false;
     }
 
     const size_t nu = b.units.size();

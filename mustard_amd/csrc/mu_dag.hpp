@@ -41,6 +41,9 @@ struct TaskRec {
     std::vector<int> deps;  // sorted unique task ids
     std::vector<OpRec> ops;
     int tag = 0;            // problem id (batched runs)
+    // Nested sub-DAG run after `ops` (insertSubgraph): tasks numbered within
+    // `sub`, their deps refer to earlier entries of `sub`.
+    std::vector<TaskRec> sub;
 };
 
 class Dag {
@@ -50,6 +53,9 @@ class Dag {
     int end_task();
     int add_op(const OpRec &op);
     void add_edge(int src, int dst);  // insertDependencyKernel (mustard.h:197-211)
+    // insertSubgraph (mustard.h:214-248): the tasks of `child` become a
+    // nested sub-DAG of the last ended task, run after its own operations.
+    int insert_sub(const Dag &child);
     int current() const { return cur_; }
     void set_tag(int tag) { tag_ = tag; }
 

@@ -16,7 +16,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -242,51 +244,78 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     HK(hipStreamSynchronize(s));
     const double eps = std::ldexp(1.0, -53);
     const double cte = res->anorm * eps * std::sqrt((double)n);
+    // a non-finite norm makes the backward error NaN (std::max would drop it)
     auto berr = [&]() {
         double w = 0.0;
         for (int j = 0; j < nrhs; j++) {
             const double d = res->anorm * hx[j] + hb[j];
+            if (!std::isfinite(hr[j]) || !std::isfinite(d))
+                return std::numeric_limits<double>::quiet_NaN();
             w = std::max(w, d > 0.0 ? hr[j] / d : 0.0);
         }
         return w;
     };
+    // the stopping test fails on any non-finite norm (the column maxima carry
+    // NaN through), so a NaN or overflow never counts as convergence
     auto small = [&]() {
+        if (!std::isfinite(cte))
+            return false;
         for (int j = 0; j < nrhs; j++)
-            if (!(hr[j] <= hx[j] * cte))
+            if (!std::isfinite(hr[j]) || !std::isfinite(hx[j]) || !(hr[j] <= hx[j] * cte))
                 return false;
         return true;
     };
 
-    CK(factor(fx));
-    HK(hipEventRecord(e1.e, s));
-    if (res->info != 0) {
-        mu_solve_ir_impl_set_error("mu_solve_ir: the factorization failed (info > 0)");
-        return MU_ERUN;
+    // LAPACK dsgesv/dsposv: a matrix with an entry beyond the single-
+    // precision range (here ||A||_inf > FLT_MAX bounds every entry) is not
+    // rounded to FP32 / BF16 at all; it goes straight to the FP64 path
+    const bool out_of_range = opts->prec != MU_PREC_FP64 && !(res->anorm <= FLT_MAX);
+    if (out_of_range && !opts->fallback) {
+        mu_solve_ir_impl_set_error(
+            "mu_solve_ir: ||A||_inf exceeds the single-precision range; use FP64 or fallback");
+        return MU_EINVAL;
     }
-    // x0 = solve(b)
-    KK(mu::copy_matrix(B, ldb, R, n, n, nrhs, s));
-    CK(mu_exec_launch(sx.h, s));
-    KK(mu::copy_matrix(R, n, X, ldx, n, nrhs, s));
-    int it = 0;
-    for (;; it++) {
-        CK(residual());
-        const double w = berr();
-        if (it == 0)
-            res->berr0 = w;
-        res->berr = w;
-        if (it < (int)(sizeof res->berr_hist / sizeof res->berr_hist[0]))
-            res->berr_hist[it] = w;
-        if (small()) {
-            res->converged = 1;
-            break;
+    bool low_failed = false;
+    if (!out_of_range) {
+        CK(factor(fx));
+        // dsgesv/dsposv (ITER = -3): a reduced-precision factorization that
+        // breaks down (a zero pivot, or a Cholesky pivot the rounded updates
+        // drove non-positive) goes to the FP64 path when a fallback is allowed
+        low_failed = res->info != 0 && opts->prec != MU_PREC_FP64 && opts->fallback;
+        if (res->info != 0 && !low_failed) {
+            mu_solve_ir_impl_set_error("mu_solve_ir: the factorization failed (info > 0)");
+            return MU_ERUN;
         }
-        if (it >= itermax || !std::isfinite(w))
-            break;
-        // d = solve(r) in place, x += d
-        CK(mu_exec_launch(sx.h, s));
-        KK(mu::add_matrix(X, ldx, R, n, n, nrhs, s));
     }
-    res->iters = it;
+    HK(hipEventRecord(e1.e, s));
+    if (out_of_range || low_failed) {
+        res->iters = 0;
+    } else {
+        // x0 = solve(b)
+        KK(mu::copy_matrix(B, ldb, R, n, n, nrhs, s));
+        CK(mu_exec_launch(sx.h, s));
+        KK(mu::copy_matrix(R, n, X, ldx, n, nrhs, s));
+        int it = 0;
+        for (;; it++) {
+            CK(residual());
+            const double w = berr();
+            if (it == 0)
+                res->berr0 = w;
+            res->berr = w;
+            if (it < (int)(sizeof res->berr_hist / sizeof res->berr_hist[0]))
+                res->berr_hist[it] = w;
+            if (small()) {
+                res->converged = 1;
+                break;
+            }
+            if (it >= itermax || !std::isfinite(w))
+                break;
+            // d = solve(r) in place, x += d
+            CK(mu_exec_launch(sx.h, s));
+            KK(mu::add_matrix(X, ldx, R, n, n, nrhs, s));
+        }
+        res->iters = it;
+    }
     if (!res->converged && opts->fallback && opts->prec != MU_PREC_FP64) {
         // LAPACK's fallback: refactor in full precision and solve
         Dag fd2, sd2;

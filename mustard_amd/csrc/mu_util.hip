@@ -203,22 +203,27 @@ int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, 
 // Column maxima of |M| (m x n): out[j] = max_i |M(i, j)|.  Non-negative
 // doubles order like their bit patterns, so an integer atomic max combines
 // the per-thread maxima; out must be zeroed first.
+// The maximum is taken on the bit patterns of |x|: they order like the
+// values, and a NaN's pattern sorts above inf, so a NaN anywhere in the
+// column makes its maximum NaN (fmax would drop it).
 __global__ void k_colmax_abs(const double *M, size_t ld, int m, int n, unsigned long long *out)
 {
     const int j = blockIdx.y;
-    double v = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
-        v = fmax(v, fabs(M[i + j * ld]));
-    __shared__ double red[256];
+    unsigned long long v = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(M[i + j * ld]));
+        v = b > v ? b : v;
+    }
+    __shared__ unsigned long long red[256];
     red[threadIdx.x] = v;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        if ((int)threadIdx.x < w && red[threadIdx.x + w] > red[threadIdx.x])
+            red[threadIdx.x] = red[threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        atomicMax(out + j, (unsigned long long)__double_as_longlong(red[0]));
+        atomicMax(out + j, red[0]);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)

@@ -1,5 +1,6 @@
-// Host driver for the persistent scheduler's emulation under ThreadSanitizer
-// (built and run by tests/test_sched_tsan.py with g++ -fsanitize=thread).
+// Host driver for the persistent scheduler's emulation under ThreadSanitizer,
+// and for the recorder and lowering under AddressSanitizer + UBSan (built and
+// run by tests/test_sched_tsan.py with g++ -fsanitize=...).
 //
 // Records the reference's tiled LU and Cholesky task sequences
 // (lu_mustard.cu:343-535, cholesky_mustard.cu:221-400) on fake device
@@ -29,16 +30,41 @@ static void task(mu::Dag &d, mu::Tile w, std::vector<mu::Tile> r, const OpRec &o
 // Tile (i, j) of an n x n column-major matrix with tiles of b.
 static double *tile(int n, int b, int i, int j) { return fake((size_t)i * b + (size_t)j * b * n); }
 
-static void record_lu(mu::Dag &d, int n, int t)
+static void record_lu(mu::Dag &d, int n, int t, int ld = 0, size_t off = 0, int subt = 0);
+
+// Diagonal GETRF task k of a tiled LU as an empty task with the tiled LU of
+// the tile (subt x subt sub-tiles) inserted as its nested sub-DAG.
+static void nested_getrf(mu::Dag &d, int n, int b, int k, int ld, size_t off, int subt)
+{
+    d.begin_task({k, k}, {{k, k}});
+    d.end_task();
+    mu::Dag child;
+    record_lu(child, b, subt, ld, off + (size_t)k * b + (size_t)k * b * ld);
+    d.insert_sub(child);
+}
+
+// ld: leading dimension (0: n); off: element offset of the matrix; subt > 0
+// nests every diagonal tile as an inserted sub-DAG of subt x subt tiles.
+static void record_lu(mu::Dag &d, int n, int t, int ld, size_t off, int subt)
 {
     const int b = n / t;
+    if (!ld)
+        ld = n;
+    auto tile = [&](int n_, int b_, int i, int j) {
+        (void)n_;
+        return fake(off + (size_t)i * b_ + (size_t)j * b_ * ld);
+    };
     for (int k = 0; k < t; k++) {
-        OpRec g;
-        g.op = mu::OP_GETRF;
-        g.m = g.n = b;
-        g.c = tile(n, b, k, k);
-        g.ldc = n;
-        task(d, {k, k}, {{k, k}}, g);
+        if (subt > 0) {
+            nested_getrf(d, n, b, k, ld, off, subt);
+        } else {
+            OpRec g;
+            g.op = mu::OP_GETRF;
+            g.m = g.n = b;
+            g.c = tile(n, b, k, k);
+            g.ldc = ld;
+            task(d, {k, k}, {{k, k}}, g);
+        }
         for (int i = k + 1; i < t; i++) {  // A_ki <- L_kk^-1 A_ki
             OpRec s;
             s.op = mu::OP_TRSM;
@@ -47,9 +73,9 @@ static void record_lu(mu::Dag &d, int n, int t)
             s.n = b;
             s.alpha = 1.0;
             s.a = tile(n, b, k, k);
-            s.lda = n;
+            s.lda = ld;
             s.c = tile(n, b, k, i);
-            s.ldc = n;
+            s.ldc = ld;
             task(d, {k, i}, {{k, k}, {k, i}}, s);
         }
         for (int i = k + 1; i < t; i++) {
@@ -60,9 +86,9 @@ static void record_lu(mu::Dag &d, int n, int t)
             s.n = b;
             s.alpha = 1.0;
             s.a = tile(n, b, k, k);
-            s.lda = n;
+            s.lda = ld;
             s.c = tile(n, b, i, k);
-            s.ldc = n;
+            s.ldc = ld;
             task(d, {i, k}, {{k, k}, {i, k}}, s);
             for (int j = k + 1; j < t; j++) {
                 OpRec m;  // A_ij -= A_ik A_kj
@@ -71,7 +97,7 @@ static void record_lu(mu::Dag &d, int n, int t)
                 m.a = tile(n, b, i, k);
                 m.b = tile(n, b, k, j);
                 m.c = tile(n, b, i, j);
-                m.lda = m.ldb = m.ldc = n;
+                m.lda = m.ldb = m.ldc = ld;
                 task(d, {i, j}, {{i, k}, {k, j}, {i, j}}, m);
             }
         }
@@ -130,17 +156,19 @@ int main()
 {
     struct Case {
         const char *kind;
-        int n, t, wgs, delay_us, sm;
+        int n, t, wgs, delay_us, sm, subt;
     };
     const Case cases[] = {
-        {"lu", 600, 2, 3, 0, 0},     {"lu", 1200, 4, 6, 0, 0},   {"chol", 1200, 4, 6, 0, 0},
-        {"lu", 1200, 3, 4, 50, 0},   {"chol", 1500, 3, 5, 0, 20}, {"lu", 1000, 4, 8, 0, 100},
+        {"lu", 600, 2, 3, 0, 0, 0},    {"lu", 1200, 4, 6, 0, 0, 0},
+        {"chol", 1200, 4, 6, 0, 0, 0}, {"lu", 1200, 3, 4, 50, 0, 0},
+        {"chol", 1500, 3, 5, 0, 20, 0}, {"lu", 1000, 4, 8, 0, 100, 0},
+        {"lu", 2400, 2, 6, 20, 0, 3},  // diagonal tiles as inserted sub-DAGs
     };
     int bad = 0;
     for (const Case &c : cases) {
         mu::Dag d;
         if (std::string(c.kind) == "lu")
-            record_lu(d, c.n, c.t);
+            record_lu(d, c.n, c.t, 0, 0, c.subt);
         else
             record_cholesky(d, c.n, c.t);
         mu::LowerOpts lo;
@@ -162,9 +190,9 @@ int main()
                         r.err == 0 && r.done == r.nitems &&
                         (c.sm == 0 || r.max_task_workgroups <= r.task_cap) &&
                         (c.sm != 100 || r.max_tasks_running == 1);
-        std::printf("%s n=%d t=%d workgroups=%d delay=%dus sm=%d: items %u executed %llu "
-                    "dup %llu order %llu err %u task max %u cap %u tasks %u %s\n",
-                    c.kind, c.n, c.t, c.wgs, c.delay_us, c.sm, r.nitems,
+        std::printf("%s n=%d t=%d sub=%d workgroups=%d delay=%dus sm=%d: items %u executed "
+                    "%llu dup %llu order %llu err %u task max %u cap %u tasks %u %s\n",
+                    c.kind, c.n, c.t, c.subt, c.wgs, c.delay_us, c.sm, r.nitems,
                     (unsigned long long)r.executed, (unsigned long long)r.duplicates,
                     (unsigned long long)r.order_violations, r.err, r.max_task_workgroups,
                     r.task_cap, r.max_tasks_running, ok ? "ok" : "FAILED");

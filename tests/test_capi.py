@@ -168,3 +168,104 @@ def test_capture_records_instead_of_running():
     assert n.value == 2 and list(ops[:2]) == [4, 1]
     with pytest.raises(mu.MustardError):
         d.end_capture(key)
+
+
+def _record_lu_tasks(d, key, A, n, t, nested=None):
+    """The CLI's tiled LU task loop (lu_mustard.cu:343-535) recorded through
+    the tile operations on `key`; for the diagonal tiles k in `nested` the
+    GETRF task records no operation and the caller inserts a sub-DAG."""
+    b = n // t
+    esz = 8
+
+    def blk(i, j):
+        return ctypes.c_void_p(A.value + (i * b + j * b * n) * esz)
+
+    d.capture(key)
+    for k in range(t):
+        d.begin_task((k, k), [(k, k)])
+        if not (nested and k in nested):
+            mu.call("mu_dgetrf_nopiv", key, b, blk(k, k), n, None)
+        d.end_task()
+        if nested and k in nested:
+            yield k
+        for i in range(k + 1, t):
+            d.begin_task((k, i), [(k, k), (k, i)])
+            mu.call("mu_dtrsm", key, b"L", b"L", b"N", b"U", b, b, 1.0, blk(k, k), n, blk(k, i), n)
+            d.end_task()
+        for i in range(k + 1, t):
+            d.begin_task((i, k), [(k, k), (i, k)])
+            mu.call("mu_dtrsm", key, b"R", b"U", b"N", b"N", b, b, 1.0, blk(k, k), n, blk(i, k), n)
+            d.end_task()
+            for j in range(k + 1, t):
+                d.begin_task((i, j), [(i, k), (k, j), (i, j)])
+                mu.call("mu_dgemm", key, b"N", b"N", b, b, b, -1.0, blk(i, k), n, blk(k, j), n, 1.0,
+                        blk(i, j), n)
+                d.end_task()
+    d.end_capture(key)
+
+
+def test_insert_subdag_lowers_the_child_inside_the_task():
+    """insertSubgraph (mustard.h:214-248): the child's tasks run as a nested
+    sub-DAG of the last ended task.  The lowered DAG holds exactly the
+    parent's units plus the child's (item counts add up), and the scheduler
+    protocol runs it with every item once and in dependency order."""
+    n, t, subt = 9000, 2, 2
+    A = _fake()
+    key = ctypes.c_void_p(777)
+    parent = mu.Dag()
+    children = []
+    for k in _record_lu_tasks(parent, key, A, n, t, nested={0, 1}):
+        child = mu.Dag()
+        b = n // t
+        child.record_lu(ctypes.c_void_p(A.value + (k * b + k * b * n) * 8), b, subt, lda=n)
+        parent.insert_subdag(child)
+        children.append(child)
+    bare = mu.Dag()
+    for _ in _record_lu_tasks(bare, ctypes.c_void_p(778), A, n, t, nested={0, 1}):
+        pass
+    r_par = _emulate_ex(parent, 16)
+    r_bare = _emulate_ex(bare, 16)
+    r_child = [_emulate_ex(c, 16) for c in children]
+    executed, dup, viol, err, done, total = r_par[:6]
+    assert (executed, dup, viol, err, done) == (total, 0, 0, 0, total)
+    assert total == r_bare[5] + sum(r[5] for r in r_child)
+    # the inserted child may be destroyed afterwards (the parent holds a copy)
+    for c in children:
+        c.close()
+    assert _emulate_ex(parent, 4)[5] == total
+
+
+def test_insert_subdag_errors_and_nested_recording():
+    parent, child = mu.Dag(), mu.Dag()
+    with pytest.raises(mu.MustardError) as e:
+        parent.insert_subdag(child)  # no task ended yet
+    assert e.value.code == _lib.MU_ESTATE
+    key = ctypes.c_void_p(4321)
+    parent.capture(key)
+    parent.begin_task((0, 0), [(0, 0)])
+    parent.end_task()
+    # a sub-graph recorded on the same stream takes it, then hands it back
+    child.capture(key)
+    child.begin_task((0, 0), [(0, 0)])
+    mu.call("mu_dgetrf_nopiv", key, 100, _fake(), 100, None)
+    child.end_task()
+    with pytest.raises(mu.MustardError):
+        parent.end_capture(key)  # the inner recording ends first
+    child.end_capture(key)
+    parent.begin_task((0, 1), [(0, 0)])
+    mu.call("mu_dgemm", key, b"N", b"N", 8, 8, 8, -1.0, _fake(), 8, _fake(), 8, 1.0, _fake(), 8)
+    parent.end_task()
+    parent.end_capture(key)
+    n = ctypes.c_int()
+    ops = (ctypes.c_int * 4)()
+    mu.call("mu_dag_task_info", parent.h, 1, None, None, ops, 4, ctypes.byref(n))
+    assert n.value == 1 and ops[0] == 1
+    mu.call("mu_dag_task_info", child.h, 0, None, None, ops, 4, ctypes.byref(n))
+    assert n.value == 1 and ops[0] == 4
+    child.begin_task((1, 1), [])
+    with pytest.raises(mu.MustardError):
+        parent.insert_subdag(child)  # the child is still recording a task
+    child.end_task()
+    parent.insert_subdag(child)
+    with pytest.raises(mu.MustardError):
+        parent.insert_subdag(parent)
