@@ -119,6 +119,19 @@ int mu_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha, co
  * All side/uplo/trans/diag combinations. */
 int mu_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, int n,
              double alpha, const double *A, int lda, double *B, int ldb);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  mu_dtrsm with the same arguments, meaning and result
+ * (bit for bit), run at once on `stream`; the inverted diagonal blocks of a
+ * triangle of order > 128 go to a fixed device array of
+ * MU_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of a
+ * per-call allocation, so triangles of order <= 8192 are accepted and larger
+ * ones rejected with MU_EINVAL.  Calls that overlap on one GPU share that
+ * array and race; a stream being recorded into a DAG is refused
+ * (MU_ESTATE).  Allocates no device memory. */
+#define MU_NOALLOC_SCRATCH_BYTES (1u << 20)
+int mu_dtrsm_noalloc(void *stream, char side, char uplo, char transa, char diag, int m, int n,
+                     double alpha, const double *A, int lda, double *B, int ldb);
 /* cusolverDnDgetrf with devIpiv = NULL -- lu_mustard.cu:360-369.  `info_dev`
  * (device int, may be NULL) receives the 1-based column of the first zero
  * pivot through an atomic minimum: initialise it to INT_MAX (mu_info_reset). */

@@ -85,6 +85,7 @@ SIGNATURES = {
     "mu_dgemm": (_i, [_vp, _c, _c, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
     "mu_dsyrk": (_i, [_vp, _c, _c, _i, _i, _d, _vp, _i, _d, _vp, _i]),
     "mu_dtrsm": (_i, [_vp, _c, _c, _c, _c, _i, _i, _d, _vp, _i, _vp, _i]),
+    "mu_dtrsm_noalloc": (_i, [_vp, _c, _c, _c, _c, _i, _i, _d, _vp, _i, _vp, _i]),
     "mu_dgetrf_nopiv": (_i, [_vp, _i, _vp, _i, _vp]),
     "mu_dpotrf": (_i, [_vp, _c, _i, _vp, _i, _vp]),
     "mu_info_reset": (_i, [_vp, _vp]),
