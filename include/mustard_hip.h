@@ -59,6 +59,12 @@ typedef struct {
                            one task's items at once, and a task starts only while the
                            budgets of the running tasks sum to less than 100.
                            0 = no budget. */
+    int no_fuse;        /* tiled: 1 keeps every GEMM/SYRK unit of a graph node separate;
+                           0 (default) merges units of one node that tile a rectangle
+                           (or, for a Cholesky update, a lower triangle) of C into one
+                           GEMM (SYRK) over the whole region: its 64 x 64 blocks then
+                           ignore the tile boundaries (no ragged blocks at the edge of
+                           every 1000-wide tile); results are bit-identical */
 } mu_exec_opts_t;
 
 typedef struct {
@@ -263,6 +269,16 @@ int mu_record_tiled_solve(mu_dag_t dag, const double *F, int n, int t, int ldf, 
  * counterpart in the reference (one factorization per process there). */
 int mu_record_batch(mu_dag_t dag, int nprob, const int *kinds, const int *ns, double *const *As,
                     const int *ldas, const int *tiles, int *info_dev, int prec);
+/* Residuals of a batch of packed problems (no reference counterpart: the
+ * refinement of batched reduced-precision solves).  Device arrays: ns[p] the
+ * order of problem p, a_off[p] the element offset of its n x n matrix
+ * (leading dimension n) in A, v_off[p] the offset of its vectors in X, B and
+ * R.  Writes r_p = b_p - A_p x_p (FP64) to R and, per problem, norms[4p..4p+3]
+ * = ||r||_inf, ||x||_inf, ||A||_inf, ||b||_inf (NaN propagates).  One
+ * workgroup per problem, one launch. */
+int mu_batch_residual(int nprob, const int *ns, const long long *a_off, const long long *v_off,
+                      const double *A, const double *X, const double *B, double *R,
+                      double *norms, void *stream);
 
 /* ---- Mixed precision + FP64 iterative refinement ------------------------
  * No counterpart in the reference: the precision form of the tiled

@@ -4,8 +4,8 @@ Python mirror of the reference's single-GPU path: a task DAG recorded with
 the TiledGraphCreator rule (mustard/mustard.h:132-384) in the order of
 tiledLU / tiledCholesky (mustard/lu_mustard.cu:343-535,
 mustard/cholesky_mustard.cu:221-400), executed as one hipGraph of
-level-batched kernels (``--tiled``); the persistent device scheduler
-(``--subgraph``) is switched off in this build.
+level-batched kernels (``--tiled``) or as one launch of the persistent device
+scheduler (``--subgraph``).
 
 Device memory and streams come from PyTorch-ROCm; all arithmetic runs in
 ``libmustard_hip.so`` (hand-written gfx950 kernels).  Column-major matrices are
@@ -180,7 +180,7 @@ class Exec:
     """A lowered, uploaded DAG ready to launch."""
 
     def __init__(self, dag: Dag, mode=MODE_TILED, grid=0, timeout_s=10.0, trace=False,
-                 small_max=0, trsm_max=0, lookahead=True, sm=0):
+                 small_max=0, trsm_max=0, lookahead=True, sm=0, fuse=True):
         if isinstance(mode, str):
             mode = MODES[mode]
         self.mode = mode
@@ -188,7 +188,7 @@ class Exec:
         # the executable points into buffers the DAG recorded; keep it alive
         self._dag = dag
         opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, 1 if lookahead else -1,
-                        sm)
+                        sm, 0 if fuse else 1)
         call("mu_exec_create", dag.h, mode, ctypes.byref(opts), ctypes.byref(self.h))
 
     def launch(self, stream=None):

@@ -26,7 +26,8 @@ class MustardError(RuntimeError):
 class ExecOpts(ctypes.Structure):
     _fields_ = [("grid", ctypes.c_int), ("timeout_s", ctypes.c_double),
                 ("trace", ctypes.c_int), ("small_max", ctypes.c_int),
-                ("trsm_max", ctypes.c_int), ("lookahead", ctypes.c_int), ("sm", ctypes.c_int)]
+                ("trsm_max", ctypes.c_int), ("lookahead", ctypes.c_int), ("sm", ctypes.c_int),
+                ("no_fuse", ctypes.c_int)]
 
 
 class EmuOpts(ctypes.Structure):
@@ -122,6 +123,7 @@ SIGNATURES = {
     "mu_gen_cond": (_i, [_vp, _i, _i, _d, _i, _u64, _vp, _vp]),
     "mu_chol_backward": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _i, _i, _vp]),
     "mu_record_batch": (_i, [_vp, _i, _ip, _ip, ctypes.POINTER(_vp), _ip, _ip, _vp, _i]),
+    "mu_batch_residual": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mu_record_tiled_solve": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _i, _i]),
     "mu_solve_ir": (_i, [_vp, _i, _i, _vp, _i, _vp, _i, _i, ctypes.POINTER(IROpts),
                          ctypes.POINTER(IRResult), _vp]),

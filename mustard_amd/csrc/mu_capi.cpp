@@ -566,6 +566,7 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
         if (opts->sm < 0 || opts->sm > 100)
             return fail(MU_EINVAL, "mu_exec_create: sm is a percentage of the GPU (0 = none)");
         eo.sm = opts->sm;
+        eo.fuse = opts->no_fuse ? 0 : 1;
     }
     // the persistent kernel runs panel solves of order <= 256 (larger ones nest)
     // and GEMM items of 2 x 2 output blocks
@@ -624,18 +625,20 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts)
         return fail(MU_EINVAL, "mu_exec_level_ops: bad arguments");
     for (int i = 0; i < 8; i++)
         counts[i] = 0;
+    // the graph's unit table (lowered units, then merged ones) is numbered in
+    // item order: find the unit of each item
+    const std::vector<mu::UnitDesc> &U = e->gunits.empty() ? e->low.units : e->gunits;
     for (size_t k = e->level_off[level]; k < e->level_off[level + 1]; k++) {
         const uint32_t item = e->level_items[k];
-        // units are numbered in item order: find the unit of this item
-        size_t lo = 0, hi = e->low.units.size();
+        size_t lo = 0, hi = U.size();
         while (hi - lo > 1) {
             size_t mid = (lo + hi) / 2;
-            if (e->low.units[mid].item_base <= item)
+            if (U[mid].item_base <= item)
                 lo = mid;
             else
                 hi = mid;
         }
-        counts[e->low.units[lo].op % 8]++;
+        counts[U[lo].op % 8]++;
     }
     return MU_OK;
 }
@@ -1148,6 +1151,17 @@ int mu_record_batch(mu_dag_t dag, int nprob, const int *kinds, const int *ns, do
             return fail(rc, "mu_record_batch: problem " + std::to_string(p) + ": " + g_err);
     }
     return mu_dag_set_tag(dag, 0);
+}
+
+int mu_batch_residual(int nprob, const int *ns, const long long *a_off, const long long *v_off,
+                      const double *A, const double *X, const double *B, double *R,
+                      double *norms, void *stream)
+{
+    if (nprob < 0 || (nprob > 0 && (!ns || !a_off || !v_off || !A || !X || !B || !R || !norms)))
+        return fail(MU_EINVAL, "mu_batch_residual: bad arguments");
+    if (mu::batch_residual(nprob, ns, a_off, v_off, A, X, B, R, norms, (hipStream_t)stream) != 0)
+        return fail(MU_EHIP, "mu_batch_residual: kernel launch failed");
+    return MU_OK;
 }
 
 // ---- generators -------------------------------------------------------------

@@ -185,6 +185,8 @@ struct Builder {
     std::vector<std::vector<uint32_t>> succ;
     std::vector<int> has_pred;
     std::vector<size_t> ws_doubles;  // per workspace slot
+    std::vector<char> upd;           // per unit: an outer-level trailing update
+    bool cur_upd = false;            // the flag of the units lower_op adds now
     int task = 0;
     std::string err;
 
@@ -203,6 +205,7 @@ struct Builder {
         units.push_back(d);
         succ.emplace_back();
         has_pred.push_back(0);
+        upd.push_back(cur_upd ? 1 : 0);
         return (int)units.size() - 1;
     }
 
@@ -302,6 +305,16 @@ struct Builder {
         }
     };
 
+    // Blocks of a nested operation of order n whose leaves hold at most
+    // `leaf`: two levels (blocks of about nest_outer, nested again) above
+    // 2 * nest_outer, one level below.
+    int nest_parts(int n, int leaf) const
+    {
+        if (o.nest_outer > leaf && n > 2 * o.nest_outer)
+            return cdiv(n, o.nest_outer);
+        return cdiv(n, leaf);
+    }
+
     static std::vector<int> split(int n, int parts)
     {
         std::vector<int> b(parts + 1);
@@ -312,6 +325,7 @@ struct Builder {
 
     bool lower_op(const OpRec &op, Frag &f)
     {
+        cur_upd = op.update && (op.op == OP_GEMM || op.op == OP_SYRK);
         switch (op.op) {
         case OP_GEMM: {
             if (op.m <= 0 || op.n <= 0)
@@ -427,7 +441,8 @@ struct Builder {
     // (mustard/lu_mustard.cu:133-254, recordSubgraph).
     bool lower_getrf_nested(const OpRec &op, Frag &f)
     {
-        const int n = op.m, P = cdiv(n, o.small_max);
+        const int n = op.m, P = nest_parts(n, o.small_max);
+        const bool outer = P < cdiv(n, o.small_max);  // blocks that nest again
         std::vector<int> bd = split(n, P);
         const size_t ld = op.ldc;
         auto blk = [&](int i, int j) { return op.c + bd[i] + (size_t)bd[j] * ld; };
@@ -483,6 +498,7 @@ struct Builder {
                     g2.c = blk(i, j);
                     g2.ldc = (int)ld;
                     g2.flags = op.flags & (F_LOW_F32 | F_LOW_BF16);
+                    g2.update = outer;
                     if (!rec.add({i, j}, {{i, k}, {k, j}, {i, j}}, g2))
                         return false;
                 }
@@ -496,7 +512,8 @@ struct Builder {
     // (order of mustard/cholesky_mustard.cu:221-400).
     bool lower_potrf_nested(const OpRec &op, Frag &f)
     {
-        const int n = op.m, P = cdiv(n, o.small_max);
+        const int n = op.m, P = nest_parts(n, o.small_max);
+        const bool outer = P < cdiv(n, o.small_max);  // blocks that nest again
         std::vector<int> bd = split(n, P);
         const size_t ld = op.ldc;
         auto blk = [&](int i, int j) { return op.c + bd[i] + (size_t)bd[j] * ld; };
@@ -536,12 +553,14 @@ struct Builder {
                 s.c = blk(i, i);
                 s.ldc = (int)ld;
                 s.flags = op.flags & (F_LOW_F32 | F_LOW_BF16);
+                s.update = outer;
                 if (!rec.add({i, i}, {{i, i}, {i, k}}, s))
                     return false;
                 for (int j = i + 1; j < P; j++) {
                     OpRec g;
                     g.op = OP_GEMM;
                     g.flags = F_TB | (op.flags & (F_LOW_F32 | F_LOW_BF16));
+                    g.update = outer;
                     g.m = sz(j);
                     g.n = sz(i);
                     g.k = sz(k);
@@ -564,7 +583,7 @@ struct Builder {
     // looking over Z-space row blocks: Y_p = T_pp^-1 Z_p, then Z_q -= T_qp Y_p.
     bool lower_trsm_nested(const OpRec &op, Frag &f)
     {
-        const int n = op.m, P = cdiv(n, o.trsm_max);
+        const int n = op.m, P = nest_parts(n, o.trsm_max);
         std::vector<int> bd = split(n, P);
         const bool rev = op.flags & F_REV, tt = op.flags & F_T_TRANS, zr = op.flags & F_Z_ROWS;
         const size_t lda = op.lda, ldz = op.ldc;
@@ -788,6 +807,7 @@ bool lower(const Dag &dag, const LowerOpts &opts, Lowered &out, std::string &err
     out.units = std::move(b.units);
     out.succ = std::move(b.succ);
     out.ws_doubles = std::move(b.ws_doubles);
+    out.update_unit = std::move(b.upd);
     return true;
 }
 

@@ -31,6 +31,9 @@ struct OpRec {
     double *c = nullptr;
     int32_t *info = nullptr;
     int32_t info_base = 0;
+    // a trailing update at the outer level of a two-level nested
+    // factorization (runs on the bulk lane like a recorded GEMM/SYRK task)
+    bool update = false;
 };
 
 typedef std::pair<int, int> Tile;
@@ -75,6 +78,11 @@ struct LowerOpts {
     int nprio = NPRIO;
     int gemm_group = 1;  // GEMM items of gemm_group x gemm_group output blocks (the persistent
                          // kernel pays its per-item scheduling and ordering once per group)
+    // Factorizations and solves of order above 2 * nest_outer first split
+    // into blocks of about nest_outer (whose diagonal operations nest again),
+    // so a whole-matrix task lowers like the tiled DAG instead of into
+    // (n / small_max)^2 sub-tiles.
+    int nest_outer = 1000;
 };
 
 struct Lowered {
@@ -89,6 +97,7 @@ struct Lowered {
     // doubles; the executor allocates them and points the TRINV unit (c) and
     // its TRSM unit (b) of each slot there.
     std::vector<size_t> ws_doubles;
+    std::vector<char> update_unit;  // per unit: lowered from an OpRec with `update`
 };
 
 // Point the units of every workspace slot into ws (ws_total_doubles(low)

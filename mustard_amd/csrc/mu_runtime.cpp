@@ -5,6 +5,7 @@
 #include <chrono>
 #include <climits>
 #include <cstring>
+#include <map>
 
 #include "mu_kernels.hpp"
 #include "mu_runtime.hpp"
@@ -39,6 +40,201 @@ static double unit_flops(const UnitDesc &u)
 }
 
 static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------
+// Merging the GEMM/SYRK units of one graph node.  The units of a level node
+// are independent, and the trailing update of a tiled factorization is one
+// GEMM (LU) or SYRK (Cholesky) cut into 1000-wide tiles: merged, its 64 x 64
+// blocks run over the whole region instead of restarting at every tile edge,
+// where a 1000-wide tile leaves a 40-wide ragged block row and column (5.9 %
+// of the MFMA work of a tile).  A merged unit computes exactly what its parts
+// computed, element for element and in the same order (bit-identical), as
+// long as every part's operand and result addresses follow the merged
+// unit's address arithmetic, which is what the checks below establish.
+// ---------------------------------------------------------------------------
+static bool mergeable(const UnitDesc &d)
+{
+    return (d.op == OP_GEMM || d.op == OP_SYRK) && d.nitems > 0 && d.grp <= 1 &&
+           !(d.flags & (F_NARROW | F_TA | F_SRC_LOW));
+}
+
+static bool same_shape(const UnitDesc &x, const UnitDesc &y)
+{
+    return x.op == y.op && x.flags == y.flags && x.m == y.m && x.n == y.n && x.k == y.k &&
+           x.lda == y.lda && x.ldb == y.ldb && x.ldc == y.ldc && x.alpha == y.alpha &&
+           x.beta == y.beta;
+}
+
+// element offset q - p (both into double arrays)
+static int64_t eoff(const double *p, const double *q) { return (int64_t)(q - p); }
+
+struct MergeOut {
+    std::vector<UnitDesc> &table;  // graph unit table; merged units are appended
+    uint32_t &next_item;           // next free item number
+    std::vector<uint32_t> &item_unit;
+    uint32_t add(UnitDesc d)
+    {
+        d.nitems = unit_items(d);
+        d.item_base = next_item;
+        next_item += (uint32_t)d.nitems;
+        const uint32_t id = (uint32_t)table.size();
+        table.push_back(d);
+        item_unit.resize(next_item, id);
+        return id;
+    }
+};
+
+// GEMM units (same shape, op(A) = A) laid out as an R x Cn grid of C blocks:
+// unit (r, c) has a = a0 + r m, b = b0 + c n ldb (op(B) = B) or b0 + c n
+// (op(B) = B^T), c = c0 + r m + c n ldc.  Returns false unless `ids` fill the
+// grid exactly once.
+static bool merge_rect(const std::vector<UnitDesc> &U, const std::vector<uint32_t> &ids,
+                       MergeOut &mo, uint32_t &merged)
+{
+    if (ids.size() < 2)
+        return false;
+    const UnitDesc &f = U[ids[0]];
+    if (f.op != OP_GEMM)
+        return false;
+    const int64_t sb = (f.flags & F_TB) ? (int64_t)f.n : (int64_t)f.n * f.ldb;
+    const int64_t sm = f.m, sc = (int64_t)f.n * f.ldc;
+    const double *a0 = f.a, *b0 = f.b;
+    for (uint32_t u : ids) {
+        if (eoff(a0, U[u].a) < 0)
+            a0 = U[u].a;
+        if (eoff(b0, U[u].b) < 0)
+            b0 = U[u].b;
+    }
+    int64_t R = 0, C = 0;
+    std::vector<std::pair<int64_t, int64_t>> rc;
+    for (uint32_t u : ids) {
+        const int64_t da = eoff(a0, U[u].a), db = eoff(b0, U[u].b);
+        if (da % sm || db % sb)
+            return false;
+        rc.push_back({da / sm, db / sb});
+        R = std::max(R, da / sm + 1);
+        C = std::max(C, db / sb + 1);
+    }
+    if (R * C != (int64_t)ids.size() || R * f.m > INT_MAX / 2 || C * f.n > INT_MAX / 2)
+        return false;
+    std::vector<char> seen((size_t)(R * C), 0);
+    const double *c0 = U[ids[0]].c - (rc[0].first * sm + rc[0].second * sc);
+    for (size_t q = 0; q < ids.size(); q++) {
+        char &s = seen[(size_t)(rc[q].first * C + rc[q].second)];
+        if (s || U[ids[q]].c != c0 + rc[q].first * sm + rc[q].second * sc)
+            return false;
+        s = 1;
+    }
+    UnitDesc d = f;
+    d.m = (int32_t)(R * f.m);
+    d.n = (int32_t)(C * f.n);
+    d.a = a0;
+    d.b = b0;
+    d.c = const_cast<double *>(c0);
+    d.gm = (d.m + GEMM_BM - 1) / GEMM_BM;
+    d.gn = (d.n + GEMM_BN - 1) / GEMM_BN;
+    merged = mo.add(d);
+    return true;
+}
+
+// The Cholesky trailing update of rows/columns r0..r0+R-1 (tile order B):
+// SYRK units c_r = c0 + r B (1 + ldc), a_r = a0 + r B, and for every
+// j > i the GEMM (op(B) = B^T) with a = a_j, b = a_i, c = c0 + j B + i B ldc;
+// merged into one SYRK of order R B.  Removes what it merges from `ids`.
+static void merge_triangles(const std::vector<UnitDesc> &U, std::vector<uint32_t> &ids,
+                            MergeOut &mo, std::vector<uint32_t> &out)
+{
+    std::vector<uint32_t> syrk;
+    for (uint32_t u : ids)
+        if (U[u].op == OP_SYRK)
+            syrk.push_back(u);
+    if (syrk.size() < 2)
+        return;
+    std::sort(syrk.begin(), syrk.end(),
+              [&](uint32_t x, uint32_t y) { return eoff(U[x].a, U[y].a) > 0; });
+    const UnitDesc &f = U[syrk[0]];
+    const int64_t B = f.m, R = (int64_t)syrk.size();
+    for (int64_t r = 0; r < R; r++) {
+        const UnitDesc &s = U[syrk[r]];
+        if (!same_shape(s, f) || s.a != f.a + r * B || s.b != s.a ||
+            s.c != f.c + r * B * (1 + (int64_t)f.ldc) || f.lda != f.ldb)
+            return;
+    }
+    // GEMM (j, i) for every j > i, looked up by its C address
+    std::map<const double *, uint32_t> by_c;
+    for (uint32_t u : ids)
+        if (U[u].op == OP_GEMM)
+            by_c[U[u].c] = u;
+    std::vector<uint32_t> parts(syrk);
+    for (int64_t i = 0; i < R; i++)
+        for (int64_t j = i + 1; j < R; j++) {
+            auto it = by_c.find(f.c + j * B + i * B * (int64_t)f.ldc);
+            if (it == by_c.end())
+                return;
+            const UnitDesc &g = U[it->second];
+            if (g.flags != f.flags || g.m != B || g.n != B || g.k != f.k || g.lda != f.lda ||
+                g.ldb != f.lda || g.ldc != f.ldc || g.alpha != f.alpha || g.beta != f.beta ||
+                g.a != f.a + j * B || g.b != f.a + i * B)
+                return;
+            parts.push_back(it->second);
+        }
+    UnitDesc d = f;
+    d.m = d.n = (int32_t)(R * B);
+    d.gm = d.gn = (d.m + GEMM_BM - 1) / GEMM_BM;
+    out.push_back(mo.add(d));
+    std::sort(parts.begin(), parts.end());
+    std::vector<uint32_t> rest;
+    for (uint32_t u : ids)
+        if (!std::binary_search(parts.begin(), parts.end(), u))
+            rest.push_back(u);
+    ids.swap(rest);
+}
+
+// Merge what merges among the units `ids` of one GEMM node; returns the
+// node's unit list (merged units plus the ones left alone).
+static std::vector<uint32_t> merge_node(const std::vector<UnitDesc> &lowered,
+                                        std::vector<uint32_t> ids, MergeOut &mo)
+{
+    std::vector<uint32_t> out, other;
+    std::vector<uint32_t> cand;
+    for (uint32_t u : ids)
+        (mergeable(lowered[u]) ? cand : other).push_back(u);
+    merge_triangles(mo.table, cand, mo, out);
+    // groups of identical shape, then the whole group as one grid, else its
+    // rows (same a) and then its columns (same b) as strips
+    std::vector<char> used(cand.size(), 0);
+    for (size_t g = 0; g < cand.size(); g++) {
+        if (used[g])
+            continue;
+        std::vector<uint32_t> grp;
+        for (size_t h = g; h < cand.size(); h++)
+            if (!used[h] && same_shape(mo.table[cand[h]], mo.table[cand[g]])) {
+                grp.push_back(cand[h]);
+                used[h] = 1;
+            }
+        uint32_t m = 0;
+        if (merge_rect(mo.table, grp, mo, m)) {
+            out.push_back(m);
+            continue;
+        }
+        for (int pass = 0; pass < 2; pass++) {
+            std::map<const double *, std::vector<uint32_t>> lines;
+            for (uint32_t u : grp)
+                lines[pass == 0 ? mo.table[u].a : mo.table[u].b].push_back(u);
+            std::vector<uint32_t> rest;
+            for (auto &kv : lines) {
+                if (kv.second.size() >= 2 && merge_rect(mo.table, kv.second, mo, m))
+                    out.push_back(m);
+                else
+                    rest.insert(rest.end(), kv.second.begin(), kv.second.end());
+            }
+            grp.swap(rest);
+        }
+        out.insert(out.end(), grp.begin(), grp.end());
+    }
+    out.insert(out.end(), other.begin(), other.end());
+    return out;
+}
 
 int run_units(const Lowered &low, hipStream_t s, std::string &err)
 {
@@ -129,10 +325,14 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                 nlev = std::max(nlev, level[v] + 1);
             }
         // a unit belongs to a trailing update when its recorded task is a
-        // GEMM/SYRK task (not a piece of a nested factorization or solve) or
-        // the conversion of a panel tile to its reduced-precision shadow;
+        // GEMM/SYRK task (not a piece of a nested factorization or solve),
+        // when it is an outer-level update of a factorization nested in two
+        // levels (a whole-matrix task), or the conversion of a panel tile to
+        // its reduced-precision shadow;
         // such a unit whose successors are all update units is bulk work ...
         auto update_task = [&](uint32_t u) {
+            if (u < L.update_unit.size() && L.update_unit[u])
+                return true;  // outer-level update of a two-level nested factorization
             const int tk = L.units[u].task;
             if (tk < 0 || tk >= (int)dag.tasks.size() || dag.tasks[tk].ops.empty())
                 return false;
@@ -199,20 +399,32 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         std::vector<int> node_of(nkey, -1);
         std::vector<std::vector<uint32_t>> lv;
         std::vector<int> node_lane, node_level;
+        // the graph's unit table: the lowered units, then the merged ones
+        e->gunits = L.units;
+        uint32_t next_item = (uint32_t)L.nitems;
+        MergeOut mo{e->gunits, next_item, item_unit};
         {
             std::vector<std::vector<uint32_t>> by(nkey);
             for (uint32_t u = 0; u < nu; u++)
-                for (int i = 0; i < L.units[u].nitems; i++)
-                    by[ukey(u)].push_back(L.units[u].item_base + i);
+                if (L.units[u].nitems > 0)
+                    by[ukey(u)].push_back(u);
             for (size_t k = 0; k < by.size(); k++)
                 if (!by[k].empty()) {
+                    std::vector<uint32_t> us = by[k];
+                    if (eopts.fuse && k % NK == LEVEL_GEMM)
+                        us = merge_node(L.units, us, mo);
+                    std::vector<uint32_t> its;
+                    for (uint32_t u : us)
+                        for (int i = 0; i < e->gunits[u].nitems; i++)
+                            its.push_back(e->gunits[u].item_base + i);
                     node_of[k] = (int)lv.size();
                     node_lane.push_back((int)((k / NK) & 1));
                     node_level.push_back((int)(k / NK / 2));
                     e->level_gemm.push_back((char)(k % NK));
-                    lv.push_back(std::move(by[k]));
+                    lv.push_back(std::move(its));
                 }
         }
+        const uint32_t ngu = (uint32_t)e->gunits.size();
         const int nnode = (int)lv.size();
         // node of a unit; units without items (none are lowered) map to -1
         auto unode = [&](uint32_t u) { return node_of[ukey(u)]; };
@@ -255,16 +467,16 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
             o = r + bytes;
             return r;
         };
-        const size_t o_units = take(sizeof(UnitDesc) * std::max<uint32_t>(nu, 1));
-        const size_t o_iu = take(sizeof(uint32_t) * std::max<size_t>(L.nitems, 1));
+        const size_t o_units = take(sizeof(UnitDesc) * std::max<uint32_t>(ngu, 1));
+        const size_t o_iu = take(sizeof(uint32_t) * std::max<size_t>(next_item, 1));
         const size_t o_items = take(sizeof(uint32_t) * std::max<size_t>(items.size(), 1));
         e->dbytes = o;
         HIPCHK(hipMalloc(&e->dbuf, e->dbytes));
-        if (nu)
-            HIPCHK(hipMemcpy(e->dbuf + o_units, L.units.data(), sizeof(UnitDesc) * nu,
+        if (ngu)
+            HIPCHK(hipMemcpy(e->dbuf + o_units, e->gunits.data(), sizeof(UnitDesc) * ngu,
                              hipMemcpyHostToDevice));
-        if (L.nitems)
-            HIPCHK(hipMemcpy(e->dbuf + o_iu, item_unit.data(), sizeof(uint32_t) * L.nitems,
+        if (next_item)
+            HIPCHK(hipMemcpy(e->dbuf + o_iu, item_unit.data(), sizeof(uint32_t) * next_item,
                              hipMemcpyHostToDevice));
         if (!items.empty())
             HIPCHK(hipMemcpy(e->dbuf + o_items, items.data(), sizeof(uint32_t) * items.size(),

@@ -26,6 +26,7 @@ struct ExecOpts {
     int lookahead = 1;         // tiled: bulk updates on a second, concurrent chain
     int sm = 0;                // persistent: CU budget per recorded task, percent of the
                                // GPU (the reference's --sm); 0 = none
+    int fuse = 1;              // tiled: merge a node's GEMM/SYRK units that tile a region
 };
 
 
@@ -57,6 +58,7 @@ struct Exec {
     hipGraphExec_t gexec = nullptr;
     int levels = 0;                  // graph nodes (level kernels of both chains)
     int depth = 0;                   // DAG levels
+    std::vector<UnitDesc> gunits;  // tiled: the graph's unit table (lowered + merged units)
     std::vector<uint32_t> level_items;
     std::vector<size_t> level_off;
     std::vector<char> level_gemm;    // LEVEL_*: the kernel the level runs on
@@ -91,6 +93,9 @@ int sum_abs(const double *A, int lda, int m, int n, double *acc, hipStream_t s);
 int copy_matrix(const double *src, int lds, double *dst, int ldd, int m, int n, hipStream_t s);
 // out[j] = max_i |M(i, j)| for the n columns of M (m x n)
 int colmax_abs(const double *M, int ld, int m, int n, double *out, hipStream_t s);
+int batch_residual(int nprob, const int *ns, const long long *a_off, const long long *v_off,
+                   const double *A, const double *X, const double *B, double *R, double *norms,
+                   hipStream_t s);
 // *out = ||A||_inf (max row sum of |A|), through a work buffer of
 // inf_norm_work_doubles(m, n) doubles
 size_t inf_norm_work_doubles(int m, int n);

@@ -233,6 +233,66 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
         p[i] = v;
 }
 
+// Residuals of a batch of packed problems, one workgroup per problem p of
+// order n = ns[p]: A_p (n x n, leading dimension n) at A + a_off[p], x_p, b_p
+// and r_p at v_off[p] of X, B and R.  r = b - A x in FP64, and the four
+// maxima of the LAPACK stopping test, ||r||_inf, ||x||_inf, ||A||_inf and
+// ||b||_inf, into norms[4p .. 4p+3].  A thread owns rows tid, tid + 256, ...
+// and walks them along the columns (a wave reads 64 consecutive rows of a
+// column); maxima on the bit patterns of non-negative values (a NaN's
+// pattern sorts above inf, so NaN propagates).
+__global__ __launch_bounds__(256) void k_batch_residual(const int *ns, const long long *a_off,
+                                                        const long long *v_off, const double *A,
+                                                        const double *X, const double *B,
+                                                        double *R, double *norms)
+{
+    const int p = blockIdx.x, n = ns[p];
+    const double *a = A + a_off[p];
+    const double *x = X + v_off[p], *b = B + v_off[p];
+    double *r = R + v_off[p];
+    unsigned long long mr = 0, mx = 0, ma = 0, mb = 0;
+    auto bits = [](double v) { return (unsigned long long)__double_as_longlong(fabs(v)); };
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double acc = 0.0, asum = 0.0;
+        for (int j = 0; j < n; j++) {
+            const double v = a[i + (size_t)j * n];
+            acc = fma(v, x[j], acc);
+            asum += fabs(v);
+        }
+        const double ri = b[i] - acc;
+        r[i] = ri;
+        mr = max(mr, bits(ri));
+        mx = max(mx, bits(x[i]));
+        ma = max(ma, bits(asum));
+        mb = max(mb, bits(b[i]));
+    }
+    __shared__ unsigned long long red[4][256];
+    red[0][threadIdx.x] = mr;
+    red[1][threadIdx.x] = mx;
+    red[2][threadIdx.x] = ma;
+    red[3][threadIdx.x] = mb;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 4; q++)
+                red[q][threadIdx.x] = max(red[q][threadIdx.x], red[q][threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 4)
+        norms[4 * p + threadIdx.x] = __longlong_as_double((long long)red[threadIdx.x][0]);
+}
+
+int batch_residual(int nprob, const int *ns, const long long *a_off, const long long *v_off,
+                   const double *A, const double *X, const double *B, double *R, double *norms,
+                   hipStream_t s)
+{
+    if (nprob <= 0)
+        return 0;
+    hipLaunchKernelGGL(k_batch_residual, dim3(nprob), dim3(256), 0, s, ns, a_off, v_off, A, X, B,
+                       R, norms);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int colmax_abs(const double *M, int ld, int m, int n, double *out, hipStream_t s)
 {
     if (n <= 0)

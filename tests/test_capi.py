@@ -269,3 +269,20 @@ def test_insert_subdag_errors_and_nested_recording():
     parent.insert_subdag(child)
     with pytest.raises(mu.MustardError):
         parent.insert_subdag(parent)
+
+
+@pytest.mark.parametrize("kind", ["lu", "chol"])
+def test_whole_matrix_task_nests_like_the_tiled_dag(kind):
+    """A factorization of order above 2 x 1000 nests in two levels: blocks of
+    about 1000 whose diagonal operations nest again.  One whole-matrix task of
+    N = 4500 therefore lowers to exactly the units of the T = 5 tiled DAG
+    (blocks of 900), not to (4500 / 128)^2 sub-tiles, and the scheduler
+    protocol runs it."""
+    whole, tiled = mu.Dag(), mu.Dag()
+    rec = "record_lu" if kind == "lu" else "record_cholesky"
+    getattr(whole, rec)(_fake(), 4500, 1)
+    getattr(tiled, rec)(_fake(), 4500, 5)
+    r_w, r_t = _emulate_ex(whole, 16), _emulate_ex(tiled, 16)
+    assert r_w[5] == r_t[5]
+    executed, dup, viol, err, done, total = r_w[:6]
+    assert (executed, dup, viol, err, done) == (total, 0, 0, 0, total)

@@ -67,8 +67,10 @@ def test_cholesky_600_t2(torch_gpu, oracle, mode):
 
 CASES = [
     # n, t -- B = n / t; covers B = 1000 (nested GETRF/POTRF), ragged sub-tiles,
-    # B > 1024 (nested TRSM), T = 1 and B = 1
+    # B > 1024 (nested TRSM), T = 1 and B = 1, and a whole-matrix task nested in
+    # two levels (4500 -> 5 blocks of 900 -> 8 sub-tiles each)
     (3000, 3), (3000, 6), (3000, 12), (1000, 8), (130, 1), (8, 8), (4000, 2), (2100, 1),
+    (4500, 1),
 ]
 
 
@@ -274,3 +276,34 @@ def test_trsm_nesting_threshold(torch_gpu, oracle, kind, trsm_max, kernel):
     m = np.tril(np.ones((n, n), dtype=bool)) if kind == "chol" else np.ones((n, n), dtype=bool)
     err = np.max(np.abs(F[m] - ref[m]) / np.maximum(np.abs(ref[m]), 1e-3 * np.abs(ref).max()))
     assert err <= 1e-10
+
+
+@pytest.mark.parametrize("kind", ["lu", "chol"])
+@pytest.mark.parametrize("n,t,prec", [(3000, 6, 0), (2400, 4, 0), (3600, 6, 1), (1000, 1, 0)])
+def test_merged_update_units_are_bit_identical(torch_gpu, oracle, kind, n, t, prec):
+    """The tiled graph merges the GEMM/SYRK units of a node that tile one
+    rectangle (LU) or lower triangle (Cholesky) of C into one unit over the
+    whole region.  Each element is accumulated in the same order either way,
+    so the factors must be bit-identical with and without merging; the
+    merged graph runs fewer GEMM items (no ragged blocks at tile edges)."""
+    mu = _mu()
+    rng = np.random.default_rng(n + t)
+    A = oracle.gen_ref(n) if kind == "lu" else spd(rng, n)
+    out = {}
+    for fuse in (False, True):
+        dA = mu.to_device(A)
+        info = mu.info_cell(dA.device)
+        dag = mu.Dag()
+        (dag.record_lu if kind == "lu" else dag.record_cholesky)(mu.ptr(dA), n, t, None,
+                                                                 mu.ptr(info), prec)
+        ex = mu.Exec(dag, "tiled", fuse=fuse)
+        ex.launch()
+        torch_gpu.cuda.synchronize()
+        assert mu.info_value(info) == 0
+        gemm_items = sum(ex.level_items(lv) for lv in range(ex.stats()["levels"])
+                         if ex.level_info(lv)[0] == "k_batch_gemm")
+        out[fuse] = (mu.to_host(dA).copy(), gemm_items)
+    assert np.array_equal(out[False][0], out[True][0])
+    if n // t == 600:
+        # 600-wide tiles are 10 blocks of 64 each, a merged 2 x 2 region 19
+        assert out[True][1] < out[False][1]
