@@ -1,9 +1,11 @@
 """Benchmark of the north-star workload on one MI355X.
 
 Workload: tiled LU and tiled Cholesky, FP64, N = 24000 cut in 24 x 24 tiles of
-1000, each run as one hipGraph launch whose nodes are level-batched kernels
-(every independent task of one DAG level in one launch; the reference's
---tiled mode).  LU uses the reference generator (include/gen.h, glibc
+1000, each run both as one hipGraph launch whose nodes are level-batched
+kernels (every independent task of one DAG level in one launch; the
+reference's --tiled mode) and as one persistent-scheduler kernel launch (the
+reference's --subgraph mode); the headline `value` is the faster mode, and
+`modes` holds both.  LU uses the reference generator (include/gen.h, glibc
 rand, seed 420); Cholesky uses A = M M^T / N + N I with M uniform [-1, 1)
 (seed 420), built on the device.
 
@@ -15,9 +17,10 @@ value = (2N^3/3 + N^3/3) / (LU time + Cholesky time), in GFLOP/s.
 Extra objects on the JSON line: `roofline` for the dominant kernel
 (k_batch_gemm, the level-batched GEMM/SYRK kernel of the trailing updates:
 every launch of both factorizations timed with HIP events on the stream it
-runs on, in a separate pass) and
-`cpu_baseline` (the oracle's OpenMP task DAG over host BLAS, timed on a
-bounded sample on this host).
+runs on, in a separate pass), `cpu_baseline` (the oracle's OpenMP task DAG
+over host BLAS, timed on a bounded sample on this host), `mixed_ir` (A X = B
+by FP64 / FP32+IR / BF16+IR at condition numbers 1e3 and 1e6), `batched`
+(4608 uneven problems in one launch) and `single_kernel_mode` (T = 1).
 """
 from __future__ import annotations
 
@@ -87,6 +90,8 @@ def main():
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision + IR solves")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven factorizations")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-kernel (T = 1) mode factorizations")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs the tiled factorizations on a "
@@ -251,31 +256,74 @@ def main():
     # tiled solve vs FP32 (and BF16) trailing updates + refinement, each to
     # the same stopping test and reported with its backward error.
     if not args.no_mixed:
-        nrhs, kappa = 16, 1e3
-        Ac = mu.gen_cond(n, kappa, 420)
-        Bm = torch.empty((nrhs, n), dtype=torch.float64, device="cuda")
-        mu.call("mu_gen_uniform", mu.ptr(Bm), n * nrhs, 421, -1.0, 1.0, None)
-        torch.cuda.synchronize()
-        mixed = {"rhs": nrhs, "kappa": kappa, "bound_10_n_eps": 10 * n * 2.0 ** -52}
-        for kind in ("lu", "chol"):
-            r = {}
-            for prec in ("fp64", "fp32", "bf16"):
-                best = None
-                for _ in range(2):  # the first run warms the code paths
-                    _, res = mu.solve_ir(kind, Ac, Bm, n, nrhs, t, prec=prec, fallback=True)
-                    if best is None or res["total_ms"] < best["total_ms"]:
-                        best = res
-                r[prec] = {"ms": round(best["total_ms"], 3), "fact_ms": round(best["fact_ms"], 3),
-                           "iters": best["iters"], "converged": best["converged"],
-                           "fell_back": best["fell_back"], "berr": best["berr"],
-                           "berr_unrefined": best["berr0"]}
-                log(f"[bench] {kind} {prec}+IR: {best['total_ms']:.1f} ms, {best['iters']} steps, "
-                    f"berr {best['berr']:.2e}")
-            r["speedup_fp32_ir"] = round(r["fp64"]["ms"] / r["fp32"]["ms"], 3)
-            r["speedup_bf16_ir"] = round(r["fp64"]["ms"] / r["bf16"]["ms"], 3)
-            mixed[kind] = r
+        nrhs = 16
+
+        def mixed_leg(kappa):
+            Ac = mu.gen_cond(n, kappa, 420)
+            Bm = torch.empty((nrhs, n), dtype=torch.float64, device="cuda")
+            mu.call("mu_gen_uniform", mu.ptr(Bm), n * nrhs, 421, -1.0, 1.0, None)
+            torch.cuda.synchronize()
+            leg = {"rhs": nrhs, "kappa": kappa, "bound_10_n_eps": 10 * n * 2.0 ** -52}
+            for kind in ("lu", "chol"):
+                r = {}
+                for prec in ("fp64", "fp32", "bf16"):
+                    best = None
+                    for _ in range(2):  # the first run warms the code paths
+                        _, res = mu.solve_ir(kind, Ac, Bm, n, nrhs, t, prec=prec, fallback=True)
+                        if best is None or res["total_ms"] < best["total_ms"]:
+                            best = res
+                    r[prec] = {"ms": round(best["total_ms"], 3),
+                               "fact_ms": round(best["fact_ms"], 3), "iters": best["iters"],
+                               "converged": best["converged"], "fell_back": best["fell_back"],
+                               "berr": best["berr"], "berr_unrefined": best["berr0"]}
+                    log(f"[bench] kappa {kappa:g} {kind} {prec}+IR: {best['total_ms']:.1f} ms, "
+                        f"{best['iters']} steps, fell back {best['fell_back']}, "
+                        f"berr {best['berr']:.2e}")
+                r["speedup_fp32_ir"] = round(r["fp64"]["ms"] / r["fp32"]["ms"], 3)
+                r["speedup_bf16_ir"] = round(r["fp64"]["ms"] / r["bf16"]["ms"], 3)
+                leg[kind] = r
+            del Ac, Bm
+            return leg
+
+        mixed = mixed_leg(1e3)
+        # the ill-conditioned cell: kappa u_bf16 ~ 4e3, so plain BF16
+        # refinement is not expected to converge; whatever happens (refined,
+        # or the FP64 fallback) is reported with its backward error
+        mixed["kappa_1e6"] = mixed_leg(1e6)
         out["mixed_ir"] = mixed
-        del Ac, Bm
+    # The reference's single-kernel mode (no --tiled/--subgraph, T = 1): one
+    # task per factorization on the whole matrix, nested by the lowering in
+    # two levels (blocks of 1000, then sub-tiles of 125), one graph launch.
+    if not args.no_single:
+        single = {}
+        for kind, A0, A in (("lu", A_lu0, A_lu), ("cholesky", A_ch0, A_ch)):
+            dag = mu.Dag()
+            with torch.cuda.stream(stream):
+                if kind == "lu":
+                    dag.record_lu(mu.ptr(A), n, 1, None, mu.ptr(info))
+                else:
+                    dag.record_cholesky(mu.ptr(A), n, 1, None, mu.ptr(info))
+            ex = mu.Exec(dag, "tiled")
+            ms = []
+            for _ in range(2):  # warm-up, then timed
+                with torch.cuda.stream(stream):
+                    A.copy_(A0)
+                    info.fill_(2 ** 31 - 1)
+                    e0, e1 = (torch.cuda.Event(enable_timing=True),
+                              torch.cuda.Event(enable_timing=True))
+                    e0.record(stream)
+                    ex.launch(stream)
+                    e1.record(stream)
+                stream.synchronize()
+                assert mu.info_value(info) == 0
+                ms.append(e0.elapsed_time(e1))
+            fl = f_lu if kind == "lu" else f_ch
+            single[kind] = {"ms": round(ms[-1], 3),
+                            "gflops": round(fl / (ms[-1] * 1e-3) * 1e-9, 1),
+                            "units": ex.stats()["units"]}
+            log(f"[bench] single-kernel mode {kind}: {ms[-1]:.1f} ms")
+            del ex, dag
+        out["single_kernel_mode"] = single
     # Batched uneven factorizations in one graph launch: 4096 SPD Cholesky
     # problems with n uniform in [32, 512] and 512 diagonally dominant LU
     # problems with n uniform in [64, 1024] (seed 420), tile counts from the
@@ -336,6 +384,51 @@ def main():
             f"({bt.flops / (best * 1e-3) * 1e-9:.0f} GFLOP/s), {out['batched']['levels']} levels, "
             f"max scaled residual {worst:.3f}")
         bt.close()
+        # the same factorizations as one persistent-scheduler launch
+        bp = Batch(kinds, ns, tiles, mode="subgraph")
+        pms = []
+        for r in range(2):
+            bp.load(host)
+            torch.cuda.synchronize()
+            with torch.cuda.stream(stream):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                bp.launch(stream)
+                e1.record(stream)
+            stream.synchronize()
+            bp.exec.check()
+            pms.append(e0.elapsed_time(e1))
+        assert not bp.infos().any()
+        out["batched"]["persistent_ms"] = round(min(pms), 3)
+        out["batched"]["persistent_gflops"] = round(bt.flops / (min(pms) * 1e-3) * 1e-9, 1)
+        log(f"[bench] batched, one persistent launch: {min(pms):.2f} ms")
+        bp.close()
+        # A x = b for every problem: FP64 factorization + solve, and FP32 /
+        # BF16 trailing updates + FP64 refinement of all problems together;
+        # the backward error of every problem from the device residual kernel
+        from mustard_amd.batch import BatchSolve, make_rhs
+        rhs = make_rhs(ns, 421)
+        solves = {}
+        for prec in ("fp64", "fp32", "bf16"):
+            bs = BatchSolve(kinds, ns, tiles, prec=prec)
+            bs.load(host, rhs)
+            best_r = None
+            for r in range(2):  # the first run warms the code paths
+                res = bs.solve(itermax=30, stream=stream)
+                if best_r is None or res["ms"] < best_r["ms"]:
+                    best_r = res
+            assert not best_r["infos"].any()
+            solves[prec] = {"ms": round(best_r["ms"], 3), "steps": best_r["steps"],
+                            "converged": best_r["converged"],
+                            "max_berr": float(np.max(best_r["berr"])),
+                            "max_berr_unrefined": float(np.max(best_r["berr0"]))}
+            log(f"[bench] batched solve {prec}: {best_r['ms']:.2f} ms, {best_r['steps']} steps, "
+                f"max backward error {np.max(best_r['berr']):.2e} over {len(ns)} problems")
+            bs.close()
+        solves["speedup_fp32_ir"] = round(solves["fp64"]["ms"] / solves["fp32"]["ms"], 3)
+        solves["speedup_bf16_ir"] = round(solves["fp64"]["ms"] / solves["bf16"]["ms"], 3)
+        solves["berr_bound_10_nmax_eps"] = 10 * max(ns) * 2.0 ** -53
+        out["batched"]["solve"] = solves
         del host
     if not args.no_cpu:
         log("[bench] CPU baseline")

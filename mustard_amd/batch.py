@@ -5,7 +5,8 @@ items of every problem.  Tile counts per problem come from the autotuner
 (mustard_amd.autotune) or are given explicitly.
 
 Layout: the problems are packed one after another in one device buffer, each
-column-major with leading dimension equal to its order.
+column-major with leading dimension equal to its order; right-hand sides and
+solutions of BatchSolve likewise, one vector per problem.
 """
 from __future__ import annotations
 
@@ -26,9 +27,11 @@ def flops(kind, n):
 
 class Batch:
     """A batch of problems (kinds[p] in {"lu", "chol"}, orders ns[p]) on the
-    device, with one recorded DAG and one tiled executable."""
+    device, with one recorded DAG and one executable: a hipGraph of
+    level-batched kernels (mode "tiled") or one persistent-scheduler launch
+    (mode "subgraph")."""
 
-    def __init__(self, kinds, ns, tiles, prec="fp64", device="cuda"):
+    def __init__(self, kinds, ns, tiles, prec="fp64", device="cuda", mode="tiled"):
         import torch
         self.kinds, self.ns = list(kinds), [int(n) for n in ns]
         self.tiles = [int(t) for t in tiles]
@@ -48,7 +51,7 @@ class Batch:
         self.dag = Dag()
         call("mu_record_batch", self.dag.h, np_, k, nn, ps, ld, tt, ptr(self.info),
              PREC[prec] if isinstance(prec, str) else prec)
-        self.exec = Exec(self.dag, "tiled")
+        self.exec = Exec(self.dag, mode)
 
     @property
     def flops(self):
@@ -66,7 +69,9 @@ class Batch:
         self.buf.copy_(torch.from_numpy(host_buf))
 
     def launch(self, stream=None):
-        self.info.fill_(2 ** 31 - 1)
+        import torch
+        with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream()):
+            self.info.fill_(2 ** 31 - 1)
         self.exec.launch(stream)
 
     def infos(self):
@@ -80,6 +85,110 @@ class Batch:
     def close(self):
         self.exec.close()
         self.dag.close()
+
+
+class BatchSolve:
+    """A_p x_p = b_p for every problem p of a batch, by a batched
+    factorization whose trailing updates run in `prec` (FP32 or BF16; FP64
+    panels and diagonal factors), followed by FP64 iterative refinement of
+    all problems together: x = x + solve(b - A x), every step one launch of
+    the batched solve DAG (forward and back substitution tasks of every
+    problem, mu_record_tiled_solve) and one launch of the batched residual
+    kernel (mu_batch_residual), until every problem meets LAPACK dsgesv's
+    stopping test ||r||_inf <= ||x||_inf ||A||_inf 2^-53 sqrt(n) or `itermax`
+    steps have run.  The backward error of every problem,
+    ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf), comes from the device.
+
+    `mode` runs the factorization and the solves as one hipGraph ("tiled") or
+    one persistent-scheduler launch ("subgraph") each."""
+
+    def __init__(self, kinds, ns, tiles, prec="fp32", mode="tiled", device="cuda"):
+        import torch
+        self.bt = Batch(kinds, ns, tiles, prec=prec, device=device, mode=mode)
+        self.prec = prec
+        ns_ = self.bt.ns
+        self.voff = np.concatenate([[0], np.cumsum(ns_)]).astype(np.int64)
+        nv = int(self.voff[-1])
+        f64 = dict(dtype=torch.float64, device=device)
+        self.A0 = torch.empty_like(self.bt.buf)    # the matrices (the factors overwrite bt.buf)
+        self.X = torch.empty(nv, **f64)            # solution
+        self.Bv = torch.empty(nv, **f64)           # right-hand sides
+        self.R = torch.empty(nv, **f64)            # residual
+        self.S = torch.empty(nv, **f64)            # the solve DAG's vector (in place)
+        self.norms = torch.empty(4 * len(ns_), **f64)
+        self.ns_dev = torch.tensor(ns_, dtype=torch.int32, device=device)
+        self.aoff_dev = torch.tensor(self.bt.off[:-1], dtype=torch.int64, device=device)
+        self.voff_dev = torch.tensor(self.voff[:-1], dtype=torch.int64, device=device)
+        self.sdag = Dag()
+        fb, sb = self.bt.buf.data_ptr(), self.S.data_ptr()
+        for p, (k, n, t) in enumerate(zip(self.bt.kinds, ns_, self.bt.tiles)):
+            call("mu_dag_set_tag", self.sdag.h, p + 1)
+            call("mu_record_tiled_solve", self.sdag.h, ctypes.c_void_p(fb + 8 * int(self.bt.off[p])),
+                 n, t, n, KINDS[k], ctypes.c_void_p(sb + 8 * int(self.voff[p])), n, 1)
+        call("mu_dag_set_tag", self.sdag.h, 0)
+        self.sexec = Exec(self.sdag, mode)
+        self.eps_n = np.ldexp(1.0, -53) * np.sqrt(np.asarray(ns_, dtype=np.float64))
+
+    def load(self, host_A, host_b):
+        """Packed host matrices (Batch layout) and right-hand sides (packed
+        vectors, problem p at voff[p])."""
+        import torch
+        self.A0.copy_(torch.from_numpy(host_A))
+        self.Bv.copy_(torch.from_numpy(host_b))
+
+    def _residual(self, stream=None):
+        from . import stream_handle
+        call("mu_batch_residual", len(self.bt.ns), ptr(self.ns_dev), ptr(self.aoff_dev),
+             ptr(self.voff_dev), ptr(self.A0), ptr(self.X), ptr(self.Bv), ptr(self.R),
+             ptr(self.norms), stream_handle(stream))
+        nm = self.norms.view(-1, 4).cpu().numpy()
+        r, x, a, b = nm[:, 0], nm[:, 1], nm[:, 2], nm[:, 3]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            berr = np.where(a * x + b > 0, r / (a * x + b), 0.0)
+        berr[~np.isfinite(r) | ~np.isfinite(x)] = np.nan
+        ok = np.isfinite(r) & np.isfinite(x) & (r <= x * a * self.eps_n)
+        return berr, ok
+
+    def solve(self, itermax=30, stream=None):
+        """Factor, solve and refine; returns a dict with the per-problem
+        backward errors (`berr`), steps taken, whether every problem met the
+        stopping test, and the device time of the whole solve in ms."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(s):
+            self.bt.buf.copy_(self.A0)
+            e0.record(s)
+            self.bt.launch(s)
+            self.S.copy_(self.Bv)
+            self.sexec.launch(s)
+            self.X.copy_(self.S)
+            it = 0
+            while True:
+                berr, ok = self._residual(s)
+                if it == 0:
+                    berr0 = berr.copy()
+                if ok.all() or it >= itermax or not np.isfinite(berr).all():
+                    break
+                self.S.copy_(self.R)
+                self.sexec.launch(s)
+                self.X.add_(self.S)
+                it += 1
+            e1.record(s)
+        s.synchronize()
+        return {"berr": berr, "berr0": berr0, "steps": it, "converged": bool(ok.all()),
+                "ms": e0.elapsed_time(e1), "infos": self.bt.infos()}
+
+    def close(self):
+        self.sexec.close()
+        self.sdag.close()
+        self.bt.close()
+
+
+def make_rhs(ns, seed):
+    """Packed right-hand sides, uniform [-1, 1), numpy PCG64 stream `seed`."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, int(sum(ns)))
 
 
 def make_problems(kinds, ns, seed):
