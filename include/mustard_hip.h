@@ -217,7 +217,8 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
  * MU_KERNEL_TRSM_512: k_batch_trsm_mid<256 / 512>, 4-wave panel solves with
  * inverted diagonal blocks of order <= 256 / 512 and the block inversions
  * before them; MU_KERNEL_GEMM_NARROW: k_batch_gemm_narrow, GEMMs with at most
- * 16 output columns) and the algorithmic
+ * 16 output columns; MU_KERNEL_GEMM_LOW: k_batch_gemm_low, FP32 / BF16 updates
+ * reading the converted panels, both operands k-contiguous) and the algorithmic
  * flops of its units (GEMM 2mnk, SYRK n(n+1)k, TRSM m^2 n, GETRF 2m^3/3,
  * POTRF m^3/3).  Either output may be NULL. */
 #define MU_KERNEL_GENERIC 0
@@ -228,6 +229,7 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 #define MU_KERNEL_TRSM_256 5
 #define MU_KERNEL_TRSM_512 6
 #define MU_KERNEL_GEMM_NARROW 7
+#define MU_KERNEL_GEMM_LOW 8
 int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);

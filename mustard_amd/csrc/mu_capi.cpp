@@ -656,6 +656,7 @@ int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops)
                   : e->level_gemm[level] == mu::LEVEL_TRSM_256   ? MU_KERNEL_TRSM_256
                   : e->level_gemm[level] == mu::LEVEL_TRSM_512   ? MU_KERNEL_TRSM_512
                   : e->level_gemm[level] == mu::LEVEL_GEMM_NARROW ? MU_KERNEL_GEMM_NARROW
+                  : e->level_gemm[level] == mu::LEVEL_GEMM_LOW    ? MU_KERNEL_GEMM_LOW
                                                                  : MU_KERNEL_GENERIC;
     if (flops)
         *flops = e->level_flops[level];
@@ -788,17 +789,30 @@ int mu_dag_emulate_ex(mu_dag_t dag, const mu_emu_opts_t *opts, long long *result
 // operands from the shadow (F_SRC_LOW) -- half (FP32) or a quarter (BF16) of
 // the FP64 operand traffic.  C stays FP64.
 namespace {
+// Reduced-precision copies of the solved panel tiles, read by the trailing
+// updates.  Row-panel tiles (LU's U_kj, the B operand) keep the matrix's
+// layout: tile (i, j) at at(i, j), leading dimension n.  Column-panel tiles
+// (L_ik, the A operand; for Cholesky also B^T) are stored transposed, panel
+// by panel: element (global row R, k) of column panel k at
+// pt + k n b + R b + k, leading dimension b.  Both operands of an update are
+// then k-contiguous, and the tiles of one panel follow each other the way
+// one merged update reads them.
 struct Shadow {
-    char *p = nullptr;
+    char *p = nullptr, *pt = nullptr;
     size_t elem = 0;
     int ld = 0, b = 0;
     const double *at(int i, int j) const
     {
         return reinterpret_cast<const double *>(p + ((size_t)i * b + (size_t)j * b * ld) * elem);
     }
+    const double *at_t(int i, int k) const
+    {
+        return reinterpret_cast<const double *>(pt + ((size_t)k * ld * b + (size_t)i * b * b) *
+                                                         elem);
+    }
 };
 
-int make_shadow(mu_dag_t dag, int n, int b, int prec, Shadow &sh)
+int make_shadow(mu_dag_t dag, int n, int b, int prec, bool rows, Shadow &sh)
 {
     if (prec == MU_PREC_FP64)
         return MU_OK;
@@ -806,76 +820,52 @@ int make_shadow(mu_dag_t dag, int n, int b, int prec, Shadow &sh)
     sh.ld = n;
     sh.b = b;
     void *p = nullptr;
+    if (rows) {
+        HIPTRY(hipMalloc(&p, (size_t)n * n * sh.elem));
+        dag->owned->p.push_back(p);
+        sh.p = (char *)p;
+    }
     HIPTRY(hipMalloc(&p, (size_t)n * n * sh.elem));
     dag->owned->p.push_back(p);
-    sh.p = (char *)p;
+    sh.pt = (char *)p;
     return MU_OK;
 }
 
-int issue_cvt(void *key, const double *src, int lds, const Shadow &sh, int i, int j, int prec)
+// Conversion of tile (i, j) to its shadow: a row-panel tile as it is, a
+// column-panel tile (transposed) into its panel's transposed storage.
+int issue_cvt(void *key, const double *src, int lds, const Shadow &sh, int i, int j, int prec,
+              bool transposed)
 {
     mu::OpRec op;
     op.op = mu::OP_CVT;
-    op.flags = prec_flags(prec);
+    op.flags = prec_flags(prec) | (transposed ? mu::F_CVT_T : 0);
     op.m = op.n = sh.b;
     op.c = const_cast<double *>(src);
     op.ldc = lds;
-    op.b = sh.at(i, j);
-    op.ldb = sh.ld;
+    op.b = transposed ? sh.at_t(i, j) : sh.at(i, j);
+    op.ldb = transposed ? sh.b : sh.ld;
     return issue(key, op);
 }
 
+// C -= op(A) op(B) from the shadows: op(A) = the column-panel tile (ai, k)
+// read from its transposed storage (F_TA); op(B) = the row-panel tile
+// (k, bj) (LU, tb false) or the transpose of column-panel tile (bi, k)
+// (Cholesky, tb true), both k-contiguous.
 int issue_gemm_low(void *key, bool tb, const Shadow &sh, int ai, int aj, int bi, int bj,
                    double *C, int ldc, int prec)
 {
     mu::OpRec op;
     op.op = mu::OP_GEMM;
-    op.flags = (tb ? mu::F_TB : 0) | prec_flags(prec) | mu::F_SRC_LOW;
+    op.flags = mu::F_TA | prec_flags(prec) | mu::F_SRC_LOW;
     op.m = op.n = op.k = sh.b;
     op.alpha = -1.0;
     op.beta = 1.0;
-    op.a = sh.at(ai, aj);
-    op.lda = sh.ld;
-    op.b = sh.at(bi, bj);
-    op.ldb = sh.ld;
+    op.a = sh.at_t(ai, aj);
+    op.lda = sh.b;
+    op.b = tb ? sh.at_t(bi, bj) : sh.at(bi, bj);
+    op.ldb = tb ? sh.b : sh.ld;
     op.c = C;
     op.ldc = ldc;
-    return issue(key, op);
-}
-
-// Reduced-precision update reading the FP64 tiles themselves (rounded while
-// staged): the updates that feed the next panel use it, so they need not
-// wait for the shadow conversions.
-int issue_gemm_f64src(void *key, bool tb, const double *A, const double *B, int ld, double *C,
-                      int b, int prec)
-{
-    mu::OpRec op;
-    op.op = mu::OP_GEMM;
-    op.flags = (tb ? mu::F_TB : 0) | prec_flags(prec);
-    op.m = op.n = op.k = b;
-    op.alpha = -1.0;
-    op.beta = 1.0;
-    op.a = A;
-    op.lda = ld;
-    op.b = B;
-    op.ldb = ld;
-    op.c = C;
-    op.ldc = ld;
-    return issue(key, op);
-}
-
-int issue_syrk_f64src(void *key, const double *A, int ld, double *C, int b, int prec)
-{
-    mu::OpRec op;
-    op.op = mu::OP_SYRK;
-    op.flags = prec_flags(prec);
-    op.m = op.n = op.k = b;
-    op.alpha = -1.0;
-    op.beta = 1.0;
-    op.a = A;
-    op.lda = ld;
-    op.c = C;
-    op.ldc = ld;
     return issue(key, op);
 }
 
@@ -883,12 +873,12 @@ int issue_syrk_low(void *key, const Shadow &sh, int ai, int aj, double *C, int l
 {
     mu::OpRec op;
     op.op = mu::OP_SYRK;
-    op.flags = prec_flags(prec) | mu::F_SRC_LOW;
+    op.flags = mu::F_TA | prec_flags(prec) | mu::F_SRC_LOW;
     op.m = op.n = op.k = sh.b;
     op.alpha = -1.0;
     op.beta = 1.0;
-    op.a = sh.at(ai, aj);
-    op.lda = sh.ld;
+    op.a = sh.at_t(ai, aj);
+    op.lda = sh.b;
     op.c = C;
     op.ldc = ldc;
     return issue(key, op);
@@ -903,20 +893,21 @@ int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info
     void *key = dag;  // record through the same path as a captured stream
     int rc;
     Shadow sh;
-    if ((rc = make_shadow(dag, n, b, prec, sh)) != 0)
+    if ((rc = make_shadow(dag, n, b, prec, true, sh)) != 0)
         return rc;
-    const bool low = sh.p != nullptr;
+    const bool low = sh.pt != nullptr;
     if ((rc = mu_dag_capture_begin(dag, key)) != 0)
         return rc;
     auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
     auto rd = [](std::initializer_list<int> v) { return std::vector<int>(v); };
     // the shadow of tile (i, j) is written by a task of its own, tile key
-    // (i + t, j), outside the matrix's keys
+    // (i + t, j), outside the matrix's keys; L tiles (below the diagonal)
+    // are stored transposed
     auto record_cvt = [&](int i, int j) {
         const int r1[2] = {i, j};
         int c = mu_dag_begin_task(dag, i + t, j, r1, 1, nullptr);
         if (!c)
-            c = issue_cvt(key, blk(i, j), lda, sh, i, j, prec);
+            c = issue_cvt(key, blk(i, j), lda, sh, i, j, prec, i > j);
         if (!c)
             c = mu_dag_end_task(dag);
         return c;
@@ -953,15 +944,12 @@ int mu_record_tiled_lu(mu_dag_t dag, double *A, int n, int t, int lda, int *info
             if (!rc && low)
                 rc = record_cvt(i, k);
             for (int j = k + 1; j < t && rc == 0; j++) {
-                // with shadows, the updates of row and column k + 1 (the next
-                // panel's inputs) read the FP64 tiles; the others the shadows
-                const bool next = i == k + 1 || j == k + 1;
-                auto r3 = low && !next ? rd({i, j, i + t, k, k + t, j}) : rd({i, k, k, j, i, j});
+                // with shadows, every update reads the converted panel tiles
+                // (the conversions of panel k run beside the next panel's
+                // chain, so its updates wait only for them)
+                auto r3 = low ? rd({i, j, i + t, k, k + t, j}) : rd({i, k, k, j, i, j});
                 rc = mu_dag_begin_task(dag, i, j, r3.data(), 3, nullptr);
-                if (!rc && low && next)
-                    rc = issue_gemm_f64src(key, false, blk(i, k), blk(k, j), lda, blk(i, j), b,
-                                           prec);
-                else if (!rc && low)
+                if (!rc && low)
                     rc = issue_gemm_low(key, false, sh, i, k, k, j, blk(i, j), lda, prec);
                 else if (!rc)
                     rc = mu_dgemm(key, 'N', 'N', b, b, b, -1.0, blk(i, k), lda, blk(k, j), lda,
@@ -989,18 +977,19 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
     void *key = dag;
     int rc;
     Shadow sh;
-    if ((rc = make_shadow(dag, n, b, prec, sh)) != 0)
+    if ((rc = make_shadow(dag, n, b, prec, false, sh)) != 0)
         return rc;
-    const bool low = sh.p != nullptr;
+    const bool low = sh.pt != nullptr;
     if ((rc = mu_dag_capture_begin(dag, key)) != 0)
         return rc;
     auto blk = [&](int i, int j) { return A + (size_t)i * b + (size_t)j * b * lda; };
     auto rd = [](std::initializer_list<int> v) { return std::vector<int>(v); };
+    // the panel tiles' shadows, stored transposed
     auto record_cvt = [&](int i, int j) {
         const int r1[2] = {i, j};
         int c = mu_dag_begin_task(dag, i + t, j, r1, 1, nullptr);
         if (!c)
-            c = issue_cvt(key, blk(i, j), lda, sh, i, j, prec);
+            c = issue_cvt(key, blk(i, j), lda, sh, i, j, prec, true);
         if (!c)
             c = mu_dag_end_task(dag);
         return c;
@@ -1027,26 +1016,19 @@ int mu_record_tiled_cholesky(mu_dag_t dag, double *A, int n, int t, int lda, int
                 rc = record_cvt(i, k);
         }
         for (int i = k + 1; i < t && rc == 0; i++) {
-            // with shadows, the updates of column k + 1 (the next panel's
-            // inputs) read the FP64 tiles; the others the shadows
-            const bool next = i == k + 1;
-            auto r = low && !next ? rd({i, i, i + t, k}) : rd({i, i, i, k});
+            // with shadows, every update reads the converted panel tiles
+            auto r = low ? rd({i, i, i + t, k}) : rd({i, i, i, k});
             rc = mu_dag_begin_task(dag, i, i, r.data(), 2, nullptr);
-            if (!rc && low && next)
-                rc = issue_syrk_f64src(key, blk(i, k), lda, blk(i, i), b, prec);
-            else if (!rc && low)
+            if (!rc && low)
                 rc = issue_syrk_low(key, sh, i, k, blk(i, i), lda, prec);
             else if (!rc)
                 rc = mu_dsyrk(key, 'L', 'N', b, b, -1.0, blk(i, k), lda, 1.0, blk(i, i), lda);
             if (!rc)
                 rc = mu_dag_end_task(dag);
             for (int j = i + 1; j < t && rc == 0; j++) {
-                auto r3 = low && !next ? rd({j, i, j + t, k, i + t, k}) : rd({j, i, j, k, i, k});
+                auto r3 = low ? rd({j, i, j + t, k, i + t, k}) : rd({j, i, j, k, i, k});
                 rc = mu_dag_begin_task(dag, j, i, r3.data(), 3, nullptr);
-                if (!rc && low && next)
-                    rc = issue_gemm_f64src(key, true, blk(j, k), blk(i, k), lda, blk(j, i), b,
-                                           prec);
-                else if (!rc && low)
+                if (!rc && low)
                     rc = issue_gemm_low(key, true, sh, j, k, i, k, blk(j, i), lda, prec);
                 else if (!rc)
                     rc = mu_dgemm(key, 'N', 'T', b, b, b, -1.0, blk(j, k), lda, blk(i, k), lda,

@@ -169,6 +169,19 @@ static bool pair_aligned(const void *p, int flags)
     return (flags & F_LOW_BF16) ? (a & 3) == 0 : (a & 7) == 0;
 }
 
+// Reduced-precision operands both k-contiguous (op(A) = A^T of a transposed
+// panel, op(B) = B) with 16-byte aligned rows and k a whole number of
+// 16-byte chunks: the long-k-slice kernel's layout (F_KVEC).
+static bool kvec(int flags, const void *a, int lda, const void *b, int ldb, int k)
+{
+    if (!(flags & F_SRC_LOW) || !(flags & F_TA) || (flags & F_TB) ||
+        !(flags & (F_LOW_F32 | F_LOW_BF16)))
+        return false;
+    const int ve = (flags & F_LOW_BF16) ? 8 : 4;  // elements per 16 bytes
+    return k % ve == 0 && lda % ve == 0 && ldb % ve == 0 && ((uintptr_t)a & 15) == 0 &&
+           ((uintptr_t)b & 15) == 0;
+}
+
 namespace {
 
 struct Frag {
@@ -344,6 +357,8 @@ struct Builder {
             if (pair_aligned(op.a, op.flags) && pair_aligned(op.b, op.flags) && op.lda % 2 == 0 &&
                 op.ldb % 2 == 0 && op.m % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;
+            if (kvec(op.flags, op.a, op.lda, op.b, op.ldb, op.k))
+                d.flags |= F_KVEC;
             d.grp = o.gemm_group > 1 ? o.gemm_group : 1;
             int u = add_unit(d);
             f.src = {u};
@@ -359,9 +374,14 @@ struct Builder {
             d.ldb = op.lda;
             d.gm = cdiv(op.n, GEMM_BM);
             d.gn = d.gm;
-            d.flags |= F_TB;
+            // A A^T: op(B) = A^T, or, for a converted panel stored transposed
+            // (F_TA: op(A) = A^T of the stored A^T), op(B) = the stored A^T
+            if (!(op.flags & F_TA))
+                d.flags |= F_TB;
             if (pair_aligned(op.a, op.flags) && op.lda % 2 == 0 && op.n % 2 == 0 && op.k % 2 == 0)
                 d.flags |= F_VEC;
+            if (kvec(op.flags, op.a, op.lda, op.a, op.lda, op.k))
+                d.flags |= F_KVEC;
             int u = add_unit(d);
             f.src = {u};
             f.snk = {u};

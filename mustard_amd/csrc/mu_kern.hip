@@ -41,6 +41,18 @@ k_batch_small(const UnitDesc *units, const uint32_t *item_unit, const uint32_t *
     run_small_item<4>(d, (int)(item - d.item_base), smem);
 }
 
+// Reduced-precision GEMM/SYRK units whose operands are k-contiguous in
+// their storage precision (the trailing updates of an FP32 / BF16
+// factorization reading the converted panels): long k-slices, 36 KB of LDS.
+__global__ MU_GEMM_KERNEL void k_batch_gemm_low(const UnitDesc *units, const uint32_t *item_unit,
+                                                const uint32_t *items)
+{
+    __shared__ double smem[LDS_GEMM_LOWK];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    run_gemm_lowk_item(d, (int)(item - d.item_base), smem);
+}
+
 // GEMMs with at most 16 output columns (substitution steps of a solve).
 __global__ __launch_bounds__(WG_THREADS) void k_batch_gemm_narrow(const UnitDesc *units,
                                                                   const uint32_t *item_unit,
@@ -89,6 +101,7 @@ const void *level_kernel(int kind)
     case LEVEL_SMALL: return reinterpret_cast<const void *>(k_batch_small);
     case LEVEL_TRSM_SMALL: return reinterpret_cast<const void *>(k_batch_trsm_small);
     case LEVEL_GEMM_NARROW: return reinterpret_cast<const void *>(k_batch_gemm_narrow);
+    case LEVEL_GEMM_LOW: return reinterpret_cast<const void *>(k_batch_gemm_low);
     case LEVEL_TRSM_256: return reinterpret_cast<const void *>(k_batch_trsm_mid<256>);
     case LEVEL_TRSM_512: return reinterpret_cast<const void *>(k_batch_trsm_mid<512>);
     default: return generic_kernel(kind);

@@ -54,8 +54,20 @@ static size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // ---------------------------------------------------------------------------
 static bool mergeable(const UnitDesc &d)
 {
-    return (d.op == OP_GEMM || d.op == OP_SYRK) && d.nitems > 0 && d.grp <= 1 &&
-           !(d.flags & (F_NARROW | F_TA | F_SRC_LOW));
+    // FP64 storage with op(A) = A, or converted panels with op(A) = A^T (F_KVEC)
+    if (!(d.op == OP_GEMM || d.op == OP_SYRK) || d.nitems <= 0 || d.grp > 1 || (d.flags & F_NARROW))
+        return false;
+    if (d.flags & F_SRC_LOW)
+        return (d.flags & F_KVEC) != 0;
+    return !(d.flags & F_TA);
+}
+
+// storage bytes of one operand element
+static int64_t elem_bytes(const UnitDesc &d)
+{
+    if (!(d.flags & F_SRC_LOW))
+        return 8;
+    return (d.flags & F_LOW_BF16) ? 2 : 4;
 }
 
 static bool same_shape(const UnitDesc &x, const UnitDesc &y)
@@ -65,8 +77,17 @@ static bool same_shape(const UnitDesc &x, const UnitDesc &y)
            x.beta == y.beta;
 }
 
-// element offset q - p (both into double arrays)
-static int64_t eoff(const double *p, const double *q) { return (int64_t)(q - p); }
+// offset q - p in elements of `es` bytes (-1 when not a whole number of them;
+// callers compare offsets only between operands of one storage type)
+static int64_t eoff(const double *p, const double *q, int64_t es = 8)
+{
+    const int64_t d = (int64_t)((const char *)q - (const char *)p);
+    return d % es ? INT64_MIN : d / es;
+}
+static const double *eadd(const double *p, int64_t elems, int64_t es)
+{
+    return (const double *)((const char *)p + elems * es);
+}
 
 struct MergeOut {
     std::vector<UnitDesc> &table;  // graph unit table; merged units are appended
@@ -84,10 +105,11 @@ struct MergeOut {
     }
 };
 
-// GEMM units (same shape, op(A) = A) laid out as an R x Cn grid of C blocks:
-// unit (r, c) has a = a0 + r m, b = b0 + c n ldb (op(B) = B) or b0 + c n
-// (op(B) = B^T), c = c0 + r m + c n ldc.  Returns false unless `ids` fill the
-// grid exactly once.
+// GEMM units (same shape) laid out as an R x Cn grid of C blocks: unit
+// (r, c) has a = a0 + r sa with sa = m (op(A) = A) or m lda (op(A) = A^T),
+// b = b0 + c sb with sb = n ldb (op(B) = B) or n (op(B) = B^T), and
+// c = c0 + r m + c n ldc, offsets in elements of the operands' storage.
+// Returns false unless `ids` fill the grid exactly once.
 static bool merge_rect(const std::vector<UnitDesc> &U, const std::vector<uint32_t> &ids,
                        MergeOut &mo, uint32_t &merged)
 {
@@ -96,23 +118,28 @@ static bool merge_rect(const std::vector<UnitDesc> &U, const std::vector<uint32_
     const UnitDesc &f = U[ids[0]];
     if (f.op != OP_GEMM)
         return false;
+    const int64_t es = elem_bytes(f);
+    const int64_t sa = (f.flags & F_TA) ? (int64_t)f.m * f.lda : (int64_t)f.m;
     const int64_t sb = (f.flags & F_TB) ? (int64_t)f.n : (int64_t)f.n * f.ldb;
     const int64_t sm = f.m, sc = (int64_t)f.n * f.ldc;
     const double *a0 = f.a, *b0 = f.b;
     for (uint32_t u : ids) {
-        if (eoff(a0, U[u].a) < 0)
+        const int64_t da = eoff(a0, U[u].a, es), db = eoff(b0, U[u].b, es);
+        if (da == INT64_MIN || db == INT64_MIN)
+            return false;
+        if (da < 0)
             a0 = U[u].a;
-        if (eoff(b0, U[u].b) < 0)
+        if (db < 0)
             b0 = U[u].b;
     }
     int64_t R = 0, C = 0;
     std::vector<std::pair<int64_t, int64_t>> rc;
     for (uint32_t u : ids) {
-        const int64_t da = eoff(a0, U[u].a), db = eoff(b0, U[u].b);
-        if (da % sm || db % sb)
+        const int64_t da = eoff(a0, U[u].a, es), db = eoff(b0, U[u].b, es);
+        if (da == INT64_MIN || db == INT64_MIN || da % sa || db % sb)
             return false;
-        rc.push_back({da / sm, db / sb});
-        R = std::max(R, da / sm + 1);
+        rc.push_back({da / sa, db / sb});
+        R = std::max(R, da / sa + 1);
         C = std::max(C, db / sb + 1);
     }
     if (R * C != (int64_t)ids.size() || R * f.m > INT_MAX / 2 || C * f.n > INT_MAX / 2)
@@ -138,9 +165,10 @@ static bool merge_rect(const std::vector<UnitDesc> &U, const std::vector<uint32_
 }
 
 // The Cholesky trailing update of rows/columns r0..r0+R-1 (tile order B):
-// SYRK units c_r = c0 + r B (1 + ldc), a_r = a0 + r B, and for every
-// j > i the GEMM (op(B) = B^T) with a = a_j, b = a_i, c = c0 + j B + i B ldc;
-// merged into one SYRK of order R B.  Removes what it merges from `ids`.
+// SYRK units c_r = c0 + r B (1 + ldc), a_r = a0 + r sa (sa = B, or B lda
+// for a panel stored transposed, op(A) = A^T), and for every j > i the GEMM
+// with a = a_j, b = a_i, c = c0 + j B + i B ldc; merged into one SYRK of
+// order R B.  Removes what it merges from `ids`.
 static void merge_triangles(const std::vector<UnitDesc> &U, std::vector<uint32_t> &ids,
                             MergeOut &mo, std::vector<uint32_t> &out)
 {
@@ -150,13 +178,15 @@ static void merge_triangles(const std::vector<UnitDesc> &U, std::vector<uint32_t
             syrk.push_back(u);
     if (syrk.size() < 2)
         return;
+    const int64_t es = elem_bytes(U[syrk[0]]);
     std::sort(syrk.begin(), syrk.end(),
-              [&](uint32_t x, uint32_t y) { return eoff(U[x].a, U[y].a) > 0; });
+              [&](uint32_t x, uint32_t y) { return eoff(U[x].a, U[y].a, es) > 0; });
     const UnitDesc &f = U[syrk[0]];
     const int64_t B = f.m, R = (int64_t)syrk.size();
+    const int64_t sa = (f.flags & F_TA) ? B * f.lda : B;
     for (int64_t r = 0; r < R; r++) {
         const UnitDesc &s = U[syrk[r]];
-        if (!same_shape(s, f) || s.a != f.a + r * B || s.b != s.a ||
+        if (!same_shape(s, f) || s.a != eadd(f.a, r * sa, es) || s.b != s.a ||
             s.c != f.c + r * B * (1 + (int64_t)f.ldc) || f.lda != f.ldb)
             return;
     }
@@ -174,7 +204,7 @@ static void merge_triangles(const std::vector<UnitDesc> &U, std::vector<uint32_t
             const UnitDesc &g = U[it->second];
             if (g.flags != f.flags || g.m != B || g.n != B || g.k != f.k || g.lda != f.lda ||
                 g.ldb != f.lda || g.ldc != f.ldc || g.alpha != f.alpha || g.beta != f.beta ||
-                g.a != f.a + j * B || g.b != f.a + i * B)
+                g.a != eadd(f.a, j * sa, es) || g.b != eadd(f.a, i * sa, es))
                 return;
             parts.push_back(it->second);
         }
@@ -365,6 +395,16 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                     lane[u] = inner ? 1 : 0;
                 }
             }
+        // a panel tile's conversion to its reduced-precision shadow feeds the
+        // next panel's updates too: it stays on the panel lane unless every
+        // update that reads it is bulk work (unit order is topological, so
+        // its successors' lanes are settled)
+        if (eopts.lookahead)
+            for (uint32_t u = 0; u < nu; u++)
+                if (L.units[u].op == OP_CVT && lane[u] == 1)
+                    for (uint32_t v : L.succ[u])
+                        if (lane[v] == 0)
+                            lane[u] = 0;
         // the kernel a unit's items run on: GEMM/SYRK blocks on k_batch_gemm,
         // small diagonal factorizations / conversions on k_batch_small, panel
         // solves with inverted diagonal blocks (and the inversions, TRINV) on
@@ -374,6 +414,8 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         auto ukind = [&](const UnitDesc &d) -> int {
             if (d.op == OP_GEMM && (d.flags & F_NARROW))
                 return LEVEL_GEMM_NARROW;
+            if (gemm_only_op(d.op) && (d.flags & F_KVEC))
+                return LEVEL_GEMM_LOW;
             if (gemm_only_op(d.op))
                 return LEVEL_GEMM;
             if (small_only_op(d.op))
@@ -411,7 +453,7 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
             for (size_t k = 0; k < by.size(); k++)
                 if (!by[k].empty()) {
                     std::vector<uint32_t> us = by[k];
-                    if (eopts.fuse && k % NK == LEVEL_GEMM)
+                    if (eopts.fuse && (k % NK == LEVEL_GEMM || k % NK == LEVEL_GEMM_LOW))
                         us = merge_node(L.units, us, mo);
                     std::vector<uint32_t> its;
                     for (uint32_t u : us)

@@ -40,6 +40,10 @@ enum Flag : int32_t {
     F_SRC_LOW = 1 << 10, // GEMM/SYRK with F_LOW_*: a and b are stored in that precision (not FP64)
     F_NARROW = 1 << 11,  // GEMM (FP64, op(B) = B) with n <= 16: items are 16-row tiles, every
                          // column in one item
+    F_CVT_T = 1 << 12,   // CVT: the block is stored transposed, element (r, c) at b[c + r*ldb]
+    F_KVEC = 1 << 13,    // GEMM/SYRK with F_SRC_LOW, F_TA, no F_TB: both operands k-contiguous
+                         // with 16-byte aligned rows and k a multiple of 16 bytes (the
+                         // k_batch_gemm_low kernel)
 };
 
 // Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).
