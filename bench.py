@@ -1,4 +1,4 @@
-"""Benchmark of the north-star workload on one MI355X.
+"""Benchmark of tiled LU and tiled Cholesky at N = 24000 on one MI355X.
 
 Workload: tiled LU and tiled Cholesky, FP64, N = 24000 cut in 24 x 24 tiles of
 1000, each run both as one hipGraph launch whose nodes are level-batched

@@ -109,95 +109,141 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     *res = mu_ir_result_t{};
 
     // workspace: factors F (n x n), right-hand side / correction R and the
-    // forward-substitution result Y (n x nrhs each), the inverted diagonal
-    // tiles of the factors (t tiles of b x b for L, and for U with LU), the
-    // info cell and the norms
-    const int b = n / t;
+    // forward-substitution result Y (n x nrhs each), the inverses of the
+    // diagonal blocks of two tiles (DL, and DU with LU; 2b x 2b each, see
+    // below) with their b x b temporaries, the info cell and the norms
+    const int b = n / t, npair = (t + 1) / 2, b2 = 2 * b;
     const int ninv = opts->kind == MU_SOLVE_LU ? 2 : 1;
-    DevBuf dF, dR, dY, dD, dN, dI, dP;
+    DevBuf dF, dR, dY, dD, dT, dN, dI, dP;
     HK(hipMalloc(&dF.p, (size_t)n * n * sizeof(double)));
     HK(hipMalloc(&dR.p, (size_t)n * nrhs * sizeof(double)));
     HK(hipMalloc(&dY.p, (size_t)n * nrhs * sizeof(double)));
-    HK(hipMalloc(&dD.p, (size_t)ninv * n * b * sizeof(double)));
+    HK(hipMalloc(&dD.p, (size_t)ninv * npair * b2 * b2 * sizeof(double)));
+    HK(hipMalloc(&dT.p, (size_t)ninv * npair * b * b * sizeof(double)));
     HK(hipMalloc(&dN.p, (size_t)(2 * nrhs + 1) * sizeof(double)));
     HK(hipMalloc(&dI.p, sizeof(int)));
     HK(hipMalloc(&dP.p, std::max(mu::residual_partial_doubles(n, n),
                                  mu::inf_norm_work_doubles(n, n)) * sizeof(double)));
     double *F = (double *)dF.p, *R = (double *)dR.p, *Y = (double *)dY.p;
-    double *Linv = (double *)dD.p, *Uinv = Linv + (size_t)n * b;
+    double *DL = (double *)dD.p, *DU = DL + (size_t)npair * b2 * b2;
+    double *TL = (double *)dT.p, *TU = TL + (size_t)npair * b * b;
     double *rn = (double *)dN.p, *xn = rn + nrhs, *an = xn + nrhs;
     int *info = (int *)dI.p;
 
-
-    // Graphs over F, R, Y: the factorization (with the inversion of its
-    // diagonal tiles, D_k = T_kk^-1 by a TRSM against the identity, in the
-    // same graph) and the solve.  With the inverses every step of the
-    // substitutions is a GEMM over all of its output blocks instead of a
-    // single-workgroup panel solve:
-    //   forward   Y_k = Linv_k R_k;          R_i -= L_ik Y_k      (i > k)
-    //   backward  R_k = Uinv_k Y_k (LU)  or  Linv_k^T Y_k (Cholesky);
-    //             Y_i -= U_ik R_k  or  L_ki^T R_k                 (i < k)
-    // leaving the solution in R.
+    // Graphs over F, R, Y: the factorization (with the inverses of its
+    // diagonal blocks of two tiles in the same graph) and the solve.  With
+    // the inverses every step of the substitutions is a GEMM over all of its
+    // output blocks instead of a single-workgroup panel solve, and a step
+    // covers two tiles, so each substitution is a chain of t/2 steps of two
+    // GEMM levels.  Pair q holds tiles a = 2q, c = 2q + 1 (the last pair of
+    // an odd t only tile a):
+    //   DL_q = [La 0; Lca Lc]^-1 = [Da 0; X Dc],  Da = La^-1, Dc = Lc^-1,
+    //          X = -Dc (Lca Da)
+    //   DU_q = [Ua Uac; 0 Uc]^-1 = [Ea Z; 0 Ec],  Z = -Ea (Uac Ec)   (LU)
+    //   forward   Y_q = DL_q R_q;        R_i -= L_{i,q} Y_q      (tiles i after q)
+    //   backward  R_q = DU_q Y_q (LU)  or  DL_q^T Y_q (Cholesky);
+    //             Y_i -= U_{i,q} R_q  or  L_{q,i}^T R_q          (tiles i before q)
+    // leaving the solution in R.  DL_q and DU_q are 2b x 2b, leading
+    // dimension 2b, reset to the identity before every factorization; the
+    // diagonal-block inverses are TRSMs against that identity in place.
     auto fb = [&](int i, int j) { return F + (size_t)i * b + (size_t)j * b * n; };
-    auto dinv = [&](double *D, int k) { return D + (size_t)k * b * b; };
+    auto dpair = [&](double *D, int q, int r, int c) {  // block (r, c) of pair q
+        return D + (size_t)q * b2 * b2 + (size_t)r * b + (size_t)c * b * b2;
+    };
+    auto tmp = [&](double *T, int q) { return T + (size_t)q * b * b; };
+    auto width = [&](int q) { return 2 * q + 1 < t ? b2 : b; };  // rows of pair q
     auto build = [&](int prec, Dag &fd, Ex &fx, Dag &sd, Ex &sx) -> int {
         CK(mu_dag_create(&fd.h));
         if (opts->kind == MU_SOLVE_LU)
             CK(mu_record_tiled_lu(fd.h, F, n, t, n, info, prec));
         else
             CK(mu_record_tiled_cholesky(fd.h, F, n, t, n, info, prec));
-        // the inverses of the diagonal tiles join the factorization's DAG:
-        // the task for tile k reads tile (k, k) only, so it runs as soon as
-        // that tile is factored, beside the rest of the factorization.  Its
-        // output is workspace tile (k, t) (L) or (k, t + 1) (U), outside the
-        // matrix's tile keys.
+        // the inverses join the factorization's DAG: each task reads only
+        // the factored tiles it needs, so it runs beside the rest of the
+        // factorization.  Workspace keys: (tile, t) / (tile, t + 1) for the
+        // diagonal inverses of L / U, (q, t + 2) / (q, t + 3) for the
+        // temporaries, (q, t + 4) / (q, t + 5) for X / Z.
         void *key = fd.h;
+        const bool lu = opts->kind == MU_SOLVE_LU;
         CK(mu_dag_capture_begin(fd.h, key));
         for (int k = 0; k < t; k++) {
+            const int q = k / 2, o = k % 2;
             const int rd[2] = {k, k};
             CK(mu_dag_begin_task(fd.h, k, t, rd, 1, nullptr));
-            CK(mu_dtrsm(key, 'L', 'L', 'N', opts->kind == MU_SOLVE_LU ? 'U' : 'N', b, b, 1.0,
-                        fb(k, k), n, dinv(Linv, k), b));
+            CK(mu_dtrsm(key, 'L', 'L', 'N', lu ? 'U' : 'N', b, b, 1.0, fb(k, k), n,
+                        dpair(DL, q, o, o), b2));
             CK(mu_dag_end_task(fd.h));
-            if (opts->kind == MU_SOLVE_LU) {
+            if (lu) {
                 CK(mu_dag_begin_task(fd.h, k, t + 1, rd, 1, nullptr));
-                CK(mu_dtrsm(key, 'L', 'U', 'N', 'N', b, b, 1.0, fb(k, k), n, dinv(Uinv, k), b));
+                CK(mu_dtrsm(key, 'L', 'U', 'N', 'N', b, b, 1.0, fb(k, k), n,
+                            dpair(DU, q, o, o), b2));
+                CK(mu_dag_end_task(fd.h));
+            }
+        }
+        for (int q = 0; 2 * q + 1 < t; q++) {
+            const int a = 2 * q, c = a + 1;
+            // X = -Dc (Lca Da)
+            const int r1[4] = {c, a, a, t};
+            CK(mu_dag_begin_task(fd.h, q, t + 2, r1, 2, nullptr));
+            CK(mu_dgemm(key, 'N', 'N', b, b, b, 1.0, fb(c, a), n, dpair(DL, q, 0, 0), b2, 0.0,
+                        tmp(TL, q), b));
+            CK(mu_dag_end_task(fd.h));
+            const int r2[4] = {q, t + 2, c, t};
+            CK(mu_dag_begin_task(fd.h, q, t + 4, r2, 2, nullptr));
+            CK(mu_dgemm(key, 'N', 'N', b, b, b, -1.0, dpair(DL, q, 1, 1), b2, tmp(TL, q), b,
+                        0.0, dpair(DL, q, 1, 0), b2));
+            CK(mu_dag_end_task(fd.h));
+            if (lu) {
+                // Z = -Ea (Uac Ec)
+                const int r3[4] = {a, c, c, t + 1};
+                CK(mu_dag_begin_task(fd.h, q, t + 3, r3, 2, nullptr));
+                CK(mu_dgemm(key, 'N', 'N', b, b, b, 1.0, fb(a, c), n, dpair(DU, q, 1, 1), b2,
+                            0.0, tmp(TU, q), b));
+                CK(mu_dag_end_task(fd.h));
+                const int r4[4] = {q, t + 3, a, t + 1};
+                CK(mu_dag_begin_task(fd.h, q, t + 5, r4, 2, nullptr));
+                CK(mu_dgemm(key, 'N', 'N', b, b, b, -1.0, dpair(DU, q, 0, 0), b2, tmp(TU, q), b,
+                            0.0, dpair(DU, q, 0, 1), b2));
                 CK(mu_dag_end_task(fd.h));
             }
         }
         CK(mu_dag_capture_end(fd.h, key));
         CK(mu_exec_create(fd.h, MU_MODE_TILED, nullptr, &fx.h));
-        // the solve: tile k of R is (k, 0), of Y is (k, 1)
+        // the solve: tile k of R is (k, 0), of Y is (k, 1); a pair's step
+        // writes (a, 0) / (a, 1) for both its tiles
         CK(mu_dag_create(&sd.h));
         key = sd.h;
         CK(mu_dag_capture_begin(sd.h, key));
         auto rb = [&](double *M, int k) { return M + (size_t)k * b; };
-        for (int k = 0; k < t; k++) {
-            int rd0[2] = {k, 0};
-            CK(mu_dag_begin_task(sd.h, k, 1, rd0, 1, nullptr));
-            CK(mu_dgemm(key, 'N', 'N', b, nrhs, b, 1.0, dinv(Linv, k), b, rb(R, k), n, 0.0,
-                        rb(Y, k), n));
+        for (int q = 0; q < npair; q++) {
+            const int a = 2 * q, w = width(q), last = a + w / b - 1;
+            const int rd0[4] = {a, 0, last, 0};
+            CK(mu_dag_begin_task(sd.h, a, 1, rd0, 2, nullptr));
+            CK(mu_dgemm(key, 'N', 'N', w, nrhs, w, 1.0, dpair(DL, q, 0, 0), b2, rb(R, a), n, 0.0,
+                        rb(Y, a), n));
             CK(mu_dag_end_task(sd.h));
-            for (int i = k + 1; i < t; i++) {
-                int rd[4] = {k, 1, i, 0};
+            for (int i = last + 1; i < t; i++) {
+                int rd[4] = {a, 1, i, 0};
                 CK(mu_dag_begin_task(sd.h, i, 0, rd, 2, nullptr));
-                CK(mu_dgemm(key, 'N', 'N', b, nrhs, b, -1.0, fb(i, k), n, rb(Y, k), n, 1.0,
+                CK(mu_dgemm(key, 'N', 'N', b, nrhs, w, -1.0, fb(i, a), n, rb(Y, a), n, 1.0,
                             rb(R, i), n));
                 CK(mu_dag_end_task(sd.h));
             }
         }
-        const bool lu = opts->kind == MU_SOLVE_LU;
-        for (int k = t - 1; k >= 0; k--) {
-            int rd0[2] = {k, 1};
-            CK(mu_dag_begin_task(sd.h, k, 0, rd0, 1, nullptr));
-            CK(mu_dgemm(key, lu ? 'N' : 'T', 'N', b, nrhs, b, 1.0, dinv(lu ? Uinv : Linv, k), b,
-                        rb(Y, k), n, 0.0, rb(R, k), n));
+        const bool lu_ = opts->kind == MU_SOLVE_LU;
+        for (int q = npair - 1; q >= 0; q--) {
+            const int a = 2 * q, w = width(q), last = a + w / b - 1;
+            const int rd0[4] = {a, 1, last, 1};
+            CK(mu_dag_begin_task(sd.h, a, 0, rd0, 2, nullptr));
+            CK(mu_dgemm(key, lu_ ? 'N' : 'T', 'N', w, nrhs, w, 1.0,
+                        lu_ ? dpair(DU, q, 0, 0) : dpair(DL, q, 0, 0), b2, rb(Y, a), n, 0.0,
+                        rb(R, a), n));
             CK(mu_dag_end_task(sd.h));
-            for (int i = 0; i < k; i++) {
-                int rd[4] = {k, 0, i, 1};
+            for (int i = 0; i < a; i++) {
+                int rd[4] = {a, 0, i, 1};
                 CK(mu_dag_begin_task(sd.h, i, 1, rd, 2, nullptr));
-                CK(mu_dgemm(key, lu ? 'N' : 'T', 'N', b, nrhs, b, -1.0, lu ? fb(i, k) : fb(k, i), n,
-                            rb(R, k), n, 1.0, rb(Y, i), n));
+                CK(mu_dgemm(key, lu_ ? 'N' : 'T', 'N', b, nrhs, w, -1.0,
+                            lu_ ? fb(i, a) : fb(a, i), n, rb(R, a), n, 1.0, rb(Y, i), n));
                 CK(mu_dag_end_task(sd.h));
             }
         }
@@ -208,7 +254,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     auto factor = [&](Ex &fx) -> int {
         KK(mu::copy_matrix(A, lda, F, n, n, n, s));
         CK(mu_info_reset(info, s));
-        KK(mu::identity_tiles(Linv, b, t * ninv, s));
+        KK(mu::identity_tiles(DL, b2, npair * ninv, s));
         CK(mu_exec_launch(fx.h, s));
         HK(hipMemcpyAsync(&res->info, info, sizeof(int), hipMemcpyDeviceToHost, s));
         HK(hipStreamSynchronize(s));

@@ -1,4 +1,4 @@
-"""The north-star configuration itself on the GPU: tiled LU and Cholesky of
+"""The benchmark configuration itself on the GPU: tiled LU and Cholesky of
 N = 24000 in 24 x 24 tiles of 1000, leading dimension 24000, in the level-
 batched graph (--tiled) and as one persistent-scheduler launch (--subgraph).
 

@@ -59,7 +59,7 @@ def test_solve_ir(torch_gpu, oracle, kind, prec, kappa, maxit, n, t):
     backward error (<= 10 n eps) and the LAPACK solution to kappa-level
     accuracy by refinement alone: run without the FP64 fallback, so a
     refinement that stopped converging fails here.  (3000, 3) uses the
-    north-star tiles of 1000."""
+    tiles of 1000, the benchmark's tile order."""
     mu = _mu()
     nrhs = 16
     A = oracle.gen_cond(n, kappa, 7)
