@@ -138,6 +138,13 @@ int mu_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, 
 #define MU_NOALLOC_SCRATCH_BYTES (1u << 20)
 int mu_dtrsm_noalloc(void *stream, char side, char uplo, char transa, char diag, int m, int n,
                      double alpha, const double *A, int lda, double *B, int ldb);
+/* No counterpart in the reference: the substitution step of mu_solve_ir's
+ * reduced-precision refinement solves.  C = beta C + alpha A B with A an
+ * m x k FP32 matrix (column-major, lda in elements, op(A) = A), B k x n and
+ * C m x n FP64, 0 <= n <= 16: the products run in FP32 on B rounded to FP32
+ * and the sums are added into C in FP64.  Recorded or run like mu_dgemm. */
+int mu_gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc);
 /* cusolverDnDgetrf with devIpiv = NULL -- lu_mustard.cu:360-369.  `info_dev`
  * (device int, may be NULL) receives the 1-based column of the first zero
  * pivot through an atomic minimum: initialise it to INT_MAX (mu_info_reset). */

@@ -180,6 +180,47 @@ int mu::dtrsm_op(char side, char uplo, char transa, char diag, int m, int n, dou
 
 bool mu::stream_recording(void *stream) { return stream_state(stream).dag != nullptr; }
 
+int mu::gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
+                 const double *B, int ldb, double beta, double *C, int ldc)
+{
+    if (m < 0 || n < 0 || k < 0 || n > 16)
+        return fail(MU_EINVAL, "gemm_a32: need m, k >= 0 and 0 <= n <= 16");
+    if (lda < std::max(1, m) || ldb < std::max(1, k) || ldc < std::max(1, m))
+        return fail(MU_EINVAL, "gemm_a32: leading dimension too small");
+    mu::OpRec op;
+    op.op = mu::OP_GEMM;
+    op.flags = mu::F_A32;
+    op.m = m;
+    op.n = n;
+    op.k = k;
+    op.alpha = alpha;
+    op.beta = beta;
+    op.a = reinterpret_cast<const double *>(A);
+    op.lda = lda;
+    op.b = B;
+    op.ldb = ldb;
+    op.c = C;
+    op.ldc = ldc;
+    return issue(stream, op);
+}
+
+int mu::cvt_f32(void *stream, const double *src, int lds, float *dst, int ldd, int m, int n,
+                bool transposed)
+{
+    if (m < 0 || n < 0 || lds < std::max(1, m) || ldd < std::max(1, transposed ? n : m))
+        return fail(MU_EINVAL, "cvt_f32: bad sizes");
+    mu::OpRec op;
+    op.op = mu::OP_CVT;
+    op.flags = mu::F_LOW_F32 | (transposed ? mu::F_CVT_T : 0);
+    op.m = m;
+    op.n = n;
+    op.c = const_cast<double *>(src);
+    op.ldc = lds;
+    op.b = reinterpret_cast<const double *>(dst);
+    op.ldb = ldd;
+    return issue(stream, op);
+}
+
 extern "C" {
 
 const char *mu_last_error(void) { return g_err.c_str(); }
@@ -331,6 +372,12 @@ int mu_dgemm(void *stream, char transa, char transb, int m, int n, int k, double
     op.c = C;
     op.ldc = ldc;
     return issue(stream, op);
+}
+
+int mu_gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc)
+{
+    return mu::gemm_a32(stream, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
 }
 
 int mu_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha, const double *A,

@@ -17,4 +17,18 @@ int dtrsm_op(char side, char uplo, char transa, char diag, int m, int n, double 
 // Whether `stream` is being recorded into a DAG (mu_dag_capture_begin).
 bool stream_recording(void *stream);
 
+// C = beta C + alpha A B with A an m x k FP32 matrix (column-major, lda in
+// elements), B k x n and C m x n FP64, n <= 16: FP32 products on B rounded
+// to FP32, added into C in FP64 (the substitution steps of a refinement
+// solve on FP32 copies of the factors).  Recorded into the DAG capturing
+// `stream`, or run on it now.
+int gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
+             const double *B, int ldb, double beta, double *C, int ldc);
+
+// The FP64 m x n block (src, lds) rounded to FP32 into (dst, ldd); with
+// `transposed`, element (r, c) goes to dst[c + r*ldd].  Recorded or run like
+// gemm_a32.
+int cvt_f32(void *stream, const double *src, int lds, float *dst, int ldd, int m, int n,
+            bool transposed);
+
 }  // namespace mu

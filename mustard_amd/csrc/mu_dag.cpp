@@ -141,6 +141,8 @@ void patch_workspace(std::vector<UnitDesc> &units, const Lowered &low, double *w
         double *p = ws + off[d.ws_slot];
         if (d.op == OP_TRINV)
             d.c = p;
+        else if (d.op == OP_GEMM)
+            d.part = reinterpret_cast<float *>(p);
         else
             d.b = p;
     }
@@ -346,6 +348,25 @@ struct Builder {
             UnitDesc d = base(op);
             d.gm = cdiv(op.m, GEMM_BM);
             d.gn = cdiv(op.n, GEMM_BN);
+            if (op.flags & F_A32) {
+                // FP32 A, op(A) = A, op(B) = B, n <= 16 (checked where recorded)
+                d.flags |= F_NARROW;
+                // 64-row items (NARROW32_ROWS); with fewer row blocks than
+                // NARROW32_SPLIT_BELOW, split k into slices (items) as well
+                d.gm = cdiv(op.m, 64);
+                d.gn = d.gm < NARROW32_SPLIT_BELOW ? std::max(1, cdiv(op.k, NARROW32_KSLICE)) : 1;
+                if (reinterpret_cast<uintptr_t>(op.a) % 16 == 0 && op.lda % 4 == 0 && op.m % 4 == 0)
+                    d.flags |= F_VEC;
+                if (d.gn > 1) {
+                    // partial tiles (1024 floats per item) and one counter per row block
+                    d.ws_slot = (int)ws_doubles.size();
+                    ws_doubles.push_back((size_t)d.gm * d.gn * 512 + ((size_t)d.gm + 1) / 2);
+                }
+                int u = add_unit(d);
+                f.src = {u};
+                f.snk = {u};
+                return true;
+            }
             if (op.n <= 16 && !(op.flags & (F_TB | F_LOW_F32 | F_LOW_BF16))) {
                 d.flags |= F_NARROW;
                 d.gm = cdiv(op.m, 16);  // 16-row items (NARROW_ROWS)

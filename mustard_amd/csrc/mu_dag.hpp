@@ -93,9 +93,10 @@ struct Lowered {
     uint64_t nitems = 0;
     double flops = 0.0;  // algorithmic flop count of the recorded ops
     double critical_path = 0.0;
-    // Workspace of the inverted diagonal blocks: slot s holds ws_doubles[s]
-    // doubles; the executor allocates them and points the TRINV unit (c) and
-    // its TRSM unit (b) of each slot there.
+    // Workspace of the inverted diagonal blocks and of the F_A32 GEMMs split
+    // over k-slices: slot s holds ws_doubles[s] doubles; the executor
+    // allocates them, zeroed, and points the TRINV unit (c) and its TRSM unit
+    // (b), or the GEMM unit (part), of each slot there.
     std::vector<size_t> ws_doubles;
     std::vector<char> update_unit;  // per unit: lowered from an OpRec with `update`
 };

@@ -1519,10 +1519,14 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow(const UnitDesc
     const MU_G double *b = gp(u.b);
     const size_t lda = u.lda, ldb = u.ldb;
     const bool cok = li < ncol;
+    // Every load is issued unconditionally from an address clamped into the
+    // operand (a lane's row and column clamped, k clamped to K - 1), so no
+    // branch separates a load from the others in flight and the wait before
+    // a chunk's MFMAs covers only that chunk's loads.  Rows and columns past
+    // the block are computed but never stored; the k-steps past K are zeroed
+    // in both operands as they enter the MFMA.
     const MU_G double *bc = b + (size_t)min(li, ncol - 1) * ldb;
-    const int r = i0 + li;
-    const bool rok = r < u.m;
-    const int rc = min(r, u.m - 1);
+    const int rc = min(i0 + li, u.m - 1);
     const MU_G double *ar = TA ? a + (size_t)rc * lda : a + rc;
     v4d acc = {0.0, 0.0, 0.0, 0.0};
     constexpr int KU = 8;  // k-steps of 4 per chunk
@@ -1530,17 +1534,17 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow(const UnitDesc
     auto load = [&](double (&ga)[KU], double (&gb)[KU], int k) {
 #pragma unroll
         for (int q = 0; q < KU; q++) {
-            const int kk = k + 4 * q + lk;
-            const bool kok = kk < K;
-            const int kc = min(kk, K - 1);
-            gb[q] = (cok && kok) ? bc[kc] : 0.0;
-            ga[q] = (rok && kok) ? (TA ? ar[kc] : ar[(size_t)kc * lda]) : 0.0;
+            const int kc = min(k + 4 * q + lk, K - 1);
+            gb[q] = bc[kc];
+            ga[q] = TA ? ar[kc] : ar[(size_t)kc * lda];
         }
     };
-    auto mma = [&](const double (&ga)[KU], const double (&gb)[KU]) {
+    auto mma = [&](const double (&ga)[KU], const double (&gb)[KU], int k) {
 #pragma unroll
-        for (int q = 0; q < KU; q++)
-            acc = mfma_f64(ga[q], gb[q], acc);
+        for (int q = 0; q < KU; q++) {
+            const bool kok = k + 4 * q + lk < K;
+            acc = mfma_f64(kok ? ga[q] : 0.0, kok ? gb[q] : 0.0, acc);
+        }
     };
     constexpr int CH = 4 * KU;  // k per chunk
     int k = wave * CH;
@@ -1550,17 +1554,17 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow(const UnitDesc
             const int kn = k + 4 * CH;
             if (kn < K) {
                 load(fa[1], fb[1], kn);
-                mma(fa[0], fb[0]);
+                mma(fa[0], fb[0], k);
             } else {
-                mma(fa[0], fb[0]);
+                mma(fa[0], fb[0], k);
                 break;
             }
             const int kn2 = kn + 4 * CH;
             if (kn2 < K) {
                 load(fa[0], fb[0], kn2);
-                mma(fa[1], fb[1]);
+                mma(fa[1], fb[1], kn);
             } else {
-                mma(fa[1], fb[1]);
+                mma(fa[1], fb[1], kn);
                 break;
             }
             k = kn2;
@@ -1590,11 +1594,189 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow(const UnitDesc
 }
 static_assert(3 * 4 * 64 <= LDS_GEMM, "narrow GEMM reduction fits the GEMM LDS");
 
+// ---------------------------------------------------------------------------
+// Narrow GEMM with an FP32 A (F_NARROW | F_A32): C = beta C + alpha A B for
+// A m x k stored FP32 (column-major, lda in elements), B k x n FP64 (n <= 16)
+// and C FP64 -- a substitution step of a refinement solve on FP32 copies of
+// the factors, where A is the whole traffic.  An item is 64 output rows x
+// every column x one k-slice of NARROW32_KSLICE (u.gn slices): short slices
+// put many items in flight at once, so a step costs a few memory round
+// trips instead of one per 128 k.  Lane (li, lk) owns rows 4 li .. 4 li + 3
+// of the item: per k-step of 4 it loads one float4 of A (16 lanes cover 256
+// contiguous bytes of a column, 4 columns per instruction) and one element
+// B[k + lk][li], rounded to FP32.  Row e of the lane's group feeds MFMA e
+// (v_mfma_f32_16x16x4f32), whose output row index li stands for item row
+// 4 li + e.  The 4 waves split the slice in chunks of 32 taken round-robin
+// (loads of the next chunk in flight over the current chunk's MFMAs); the
+// partial tiles meet in LDS and each wave finishes 16 of the 64 rows.  One
+// slice: the FP32 sums are added into C in FP64.  Several: each item stores
+// its 64 x 16 partial tile, and the item that finds itself last of its row
+// block on the block's arrival counter adds the slices' tiles in slice order
+// (FP64; the same order whatever the arrival order, so results do not
+// depend on scheduling), updates C and resets the counter.  VEC: a 16-byte
+// aligned, lda and m multiples of 4.
+// ---------------------------------------------------------------------------
+constexpr int NARROW32_ROWS = 64;  // output rows per FP32-A narrow item
+
+template <bool VEC>
+__device__ __attribute__((always_inline)) inline void gemm_narrow_a32(const UnitDesc &u, int it,
+                                                                      double *smem)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int bi = it % u.gm, sl = it / u.gm, ns = u.gn;
+    const int i0 = bi * NARROW32_ROWS, m = u.m, ncol = u.n;
+    // this item's k range [k0, K): all of k when the unit is not split
+    const int k0 = sl * NARROW32_KSLICE, K = ns == 1 ? u.k : min(u.k, k0 + NARROW32_KSLICE);
+    const MU_G float *a = (const MU_G float *)u.a;
+    const MU_G double *b = gp(u.b);
+    const size_t lda = u.lda, ldb = u.ldb;
+    const bool cok = li < ncol;
+    // loads unconditional from clamped addresses, the k-steps past the
+    // slice zeroed as they enter the MFMA (as in gemm_narrow)
+    const MU_G double *bc = b + (size_t)min(li, ncol - 1) * ldb;
+    const int r0 = i0 + 4 * li;  // this lane's rows r0 .. r0 + 3
+    // VEC: m % 4 == 0, so the lane's 4 rows are all inside or all outside
+    const int r0c = min(r0, m - 4);
+    v4f acc[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        acc[e] = v4f{0.f, 0.f, 0.f, 0.f};
+    constexpr int KU = 8;  // k-steps of 4 per chunk
+    v4f fa[2][KU];
+    double fb[2][KU];
+    auto load = [&](v4f (&ga)[KU], double (&gb)[KU], int k) {
+#pragma unroll
+        for (int q = 0; q < KU; q++) {
+            const int kc = min(k + 4 * q + lk, K - 1);
+            gb[q] = bc[kc];
+            if (VEC) {
+                ga[q] = *(const MU_G v4f *)(a + r0c + (size_t)kc * lda);
+            } else {
+                const MU_G float *ac = a + (size_t)kc * lda;
+                ga[q] = v4f{ac[min(r0, m - 1)], ac[min(r0 + 1, m - 1)], ac[min(r0 + 2, m - 1)],
+                            ac[min(r0 + 3, m - 1)]};
+            }
+        }
+    };
+    auto mma = [&](const v4f (&ga)[KU], const double (&gb)[KU], int k) {
+#pragma unroll
+        for (int q = 0; q < KU; q++) {
+            const bool kok = k + 4 * q + lk < K;
+            const float bv = kok ? (float)gb[q] : 0.f;
+            const v4f av = kok ? ga[q] : v4f{0.f, 0.f, 0.f, 0.f};
+            acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv, acc[2], 0, 0, 0);
+            acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv, acc[3], 0, 0, 0);
+        }
+    };
+    constexpr int CH = 4 * KU;  // k per chunk
+    int k = k0 + wave * CH;
+    if (k < K) {
+        load(fa[0], fb[0], k);
+        for (;;) {
+            const int kn = k + 4 * CH;
+            if (kn < K) {
+                load(fa[1], fb[1], kn);
+                mma(fa[0], fb[0], k);
+            } else {
+                mma(fa[0], fb[0], k);
+                break;
+            }
+            const int kn2 = kn + 4 * CH;
+            if (kn2 < K) {
+                load(fa[0], fb[0], kn2);
+                mma(fa[1], fb[1], kn);
+            } else {
+                mma(fa[1], fb[1], kn);
+                break;
+            }
+            k = kn2;
+        }
+    }
+    // partial tiles to LDS: slot [wave][e][e'][lane], acc[e][e'] = D_e[4 lk + e'][li],
+    // item row 4 (4 lk + e') + e, column li
+    float *sm = reinterpret_cast<float *>(smem);
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            sm[((wave * 4 + e) * 4 + f) * 64 + lane] = acc[e][f];
+    lds_barrier();
+    // wave w finishes e' = w: rows 16 lk + 4 w + e (e = 0..3) of column li
+    float s[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        s[e] = 0.f;
+#pragma unroll
+        for (int w = 0; w < 4; w++)
+            s[e] += sm[((w * 4 + e) * 4 + wave) * 64 + lane];
+    }
+    const double alpha = u.alpha, beta = u.beta;
+    MU_G double *c = gp(u.c);
+    const int rb = i0 + 16 * lk + 4 * wave;
+    auto store = [&](const double (&v)[4]) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int row = rb + e;
+            if (row < m && cok) {
+                MU_G double *cp = c + row + (size_t)li * u.ldc;
+                *cp = (beta != 0.0 ? beta * *cp : 0.0) + alpha * v[e];
+            }
+        }
+    };
+    if (ns == 1) {
+        const double v[4] = {s[0], s[1], s[2], s[3]};
+        store(v);
+        return;
+    }
+    // several slices: publish this slice's tile, count the arrival
+    MU_G float *part = gp(u.part);
+    int32_t *cnt = reinterpret_cast<int32_t *>(u.part + (size_t)u.gm * ns * 1024) + bi;
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        part[((size_t)bi * ns + sl) * 1024 + (wave * 4 + e) * 64 + lane] = s[e];
+    // each wave waits for its stores to reach L2 and the barrier puts all of
+    // them before thread 0's agent-scope release (one L2 write-back per
+    // item) and count; the last arrival's acquire (one L2 invalidation per
+    // row block) and the barrier after it come before every wave's loads
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    int *flag = reinterpret_cast<int *>(sm + 4 * 4 * 4 * 64);
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const int old = __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == ns - 1)
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *flag = old;
+    }
+    __syncthreads();
+    if (*flag != ns - 1)
+        return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = 0; q < ns; q++)
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            v[e] += (double)part[((size_t)bi * ns + q) * 1024 + (wave * 4 + e) * 64 + lane];
+    store(v);
+    if (threadIdx.x == 0)
+        __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+static_assert(4 * 4 * 4 * 64 + 1 <= 2 * LDS_GEMM, "FP32-A narrow reduction fits the GEMM LDS");
+
 // Narrow GEMM item (F_NARROW units only; kept out of run_gemm_item so the
 // 64 x 64 block kernels keep their register budget).
 __device__ __attribute__((always_inline)) inline void run_gemm_narrow_item(const UnitDesc &u,
                                                                            int it, double *smem)
 {
+    if (u.flags & F_A32) {
+        if (u.flags & F_VEC)
+            gemm_narrow_a32<true>(u, it, smem);
+        else
+            gemm_narrow_a32<false>(u, it, smem);
+        return;
+    }
     if (u.flags & F_TA)
         gemm_narrow<true>(u, it, smem);
     else

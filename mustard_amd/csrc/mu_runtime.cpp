@@ -273,11 +273,19 @@ int run_units(const Lowered &low, hipStream_t s, std::string &err)
     const size_t wsd = ws_total_doubles(low);
     bool patched = true;
     for (const UnitDesc &d : low.units)
-        if (d.ws_slot >= 0 && ((d.op == OP_TRINV && !d.c) || (d.op == OP_TRSM && !d.b)))
+        if (d.ws_slot >= 0 && ((d.op == OP_TRINV && !d.c) || (d.op == OP_TRSM && !d.b) ||
+                               (d.op == OP_GEMM && !d.part)))
             patched = false;
     if (wsd && !patched) {
         double *ws = nullptr;
         HIPCHK(hipMallocAsync((void **)&ws, wsd * sizeof(double), s));
+        // the split GEMMs' arrival counters start at zero (a fill kernel, not
+        // a memset node, in case the stream is being captured)
+        if (fill_int(reinterpret_cast<int32_t *>(ws), wsd * 2, 0, s) != 0) {
+            (void)hipFreeAsync(ws, s);
+            err = "workspace fill failed";
+            return -1;
+        }
         std::vector<UnitDesc> units = low.units;
         patch_workspace(units, low, ws);
         Lowered copy;
@@ -323,6 +331,9 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
     HIPCHK(hipGetDevice(&e->device));
     if (const size_t wsd = ws_total_doubles(L)) {
         HIPCHK(hipMalloc((void **)&e->ws, wsd * sizeof(double)));
+        // zeroed once: the split GEMMs' arrival counters, which every launch
+        // leaves at zero again
+        HIPCHK(hipMemset(e->ws, 0, wsd * sizeof(double)));
         patch_workspace(L.units, L, e->ws);
     }
 

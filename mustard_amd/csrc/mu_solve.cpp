@@ -7,12 +7,14 @@
 // same factors, x = x + d, until ||r_j||_inf <= ||x_j||_inf ||A||_inf eps
 // sqrt(n) for every column j (eps = 2^-53, LAPACK's dlamch('E')), or ITERMAX
 // iterations; on failure LAPACK refactors in full precision, and so does this
-// driver when asked to.  Here the factors are kept in FP64 storage: only the
-// trailing GEMM/SYRK of the factorization run in FP32 or BF16 (the diagonal
-// factorizations, the panel solves, the triangular solves of the refinement,
-// the residual and the correction are FP64).  The refinement's triangular
-// solves use the inverted diagonal tiles of the factors (computed once per
-// factorization), so each substitution step is a GEMM.
+// driver when asked to.  The factors are kept in FP64 storage: the trailing
+// GEMM/SYRK of the factorization run in FP32 or BF16, the diagonal
+// factorizations and the panel solves in FP64.  As in dsgesv / dsposv the
+// refinement's triangular solves then run in FP32, on FP32 copies of the
+// factors made inside the factorization's graph; the residual and the
+// correction are FP64.  The triangular solves use the inverses of the
+// diagonal blocks of two tiles (computed once per factorization), so each
+// substitution step is a GEMM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +25,7 @@
 #include <vector>
 
 #include "../../include/mustard_hip.h"
+#include "mu_capi.hpp"
 #include "mu_runtime.hpp"
 
 namespace {
@@ -65,9 +68,6 @@ struct Ev {
 
 }  // namespace
 
-namespace mu {
-int set_error(int code, const std::string &msg);  // mu_capi.cpp: the mu_last_error() text
-}
 static void mu_solve_ir_impl_set_error(const std::string &msg) { mu::set_error(MU_EINVAL, msg); }
 
 #define CK(x)                                                                               \
@@ -129,6 +129,20 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     double *TL = (double *)dT.p, *TU = TL + (size_t)npair * b * b;
     double *rn = (double *)dN.p, *xn = rn + nrhs, *an = xn + nrhs;
     int *info = (int *)dI.p;
+    // reduced precision: FP32 copies, for the refinement's substitutions, of
+    // the factors (G, n x n: L below the diagonal blocks of two tiles, U --
+    // or L^T with Cholesky -- above them) and of the two-tile inverses (DL32,
+    // and DU32 = DU, or DL^T with Cholesky), the way dsgesv / dsposv solve
+    // with their single-precision factors
+    const size_t g_elems = ((size_t)n * n + 63) / 64 * 64, d_elems = (size_t)npair * b2 * b2;
+    DevBuf dG;
+    float *G = nullptr, *DL32 = nullptr, *DU32 = nullptr;
+    if (opts->prec != MU_PREC_FP64) {
+        HK(hipMalloc(&dG.p, (g_elems + 2 * d_elems) * sizeof(float)));
+        G = (float *)dG.p;
+        DL32 = G + g_elems;
+        DU32 = DL32 + d_elems;
+    }
 
     // Graphs over F, R, Y: the factorization (with the inverses of its
     // diagonal blocks of two tiles in the same graph) and the solve.  With
@@ -151,6 +165,8 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
         return D + (size_t)q * b2 * b2 + (size_t)r * b + (size_t)c * b * b2;
     };
     auto tmp = [&](double *T, int q) { return T + (size_t)q * b * b; };
+    auto gt = [&](int i, int j) { return G + (size_t)i * b + (size_t)j * b * n; };
+    auto d32 = [&](float *D, int q) { return D + (size_t)q * b2 * b2; };
     auto width = [&](int q) { return 2 * q + 1 < t ? b2 : b; };  // rows of pair q
     auto build = [&](int prec, Dag &fd, Ex &fx, Dag &sd, Ex &sx) -> int {
         CK(mu_dag_create(&fd.h));
@@ -207,6 +223,38 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
                 CK(mu_dag_end_task(fd.h));
             }
         }
+        if (prec != MU_PREC_FP64) {
+            // the FP32 copies, each as soon as its source is final.  Keys:
+            // (i, 2t + 6 + j) for tile (i, j) of G, (q, 3t + 6) / (q, 3t + 7)
+            // for DL32 / DU32 of pair q
+            for (int j = 0; j < t; j++)
+                for (int i = 0; i < t; i++) {
+                    if (i / 2 == j / 2 || (!lu && i < j))
+                        continue;
+                    const int rd[2] = {i, j};
+                    CK(mu_dag_begin_task(fd.h, i, 2 * t + 6 + j, rd, 1, nullptr));
+                    CK(mu::cvt_f32(key, fb(i, j), n, gt(i, j), n, b, b, false));
+                    if (!lu)  // L^T above the diagonal
+                        CK(mu::cvt_f32(key, fb(i, j), n, gt(j, i), n, b, b, true));
+                    CK(mu_dag_end_task(fd.h));
+                }
+            for (int q = 0; q < npair; q++) {
+                const int a = 2 * q, c = a + 1, nr = c < t ? 3 : 1;
+                const int rl[6] = {a, t, q, t + 4, c, t};
+                CK(mu_dag_begin_task(fd.h, q, 3 * t + 6, rl, nr, nullptr));
+                CK(mu::cvt_f32(key, dpair(DL, q, 0, 0), b2, d32(DL32, q), b2, b2, b2, false));
+                if (!lu)
+                    CK(mu::cvt_f32(key, dpair(DL, q, 0, 0), b2, d32(DU32, q), b2, b2, b2, true));
+                CK(mu_dag_end_task(fd.h));
+                if (lu) {
+                    const int ru[6] = {a, t + 1, q, t + 5, c, t + 1};
+                    CK(mu_dag_begin_task(fd.h, q, 3 * t + 7, ru, nr, nullptr));
+                    CK(mu::cvt_f32(key, dpair(DU, q, 0, 0), b2, d32(DU32, q), b2, b2, b2,
+                                   false));
+                    CK(mu_dag_end_task(fd.h));
+                }
+            }
+        }
         CK(mu_dag_capture_end(fd.h, key));
         CK(mu_exec_create(fd.h, MU_MODE_TILED, nullptr, &fx.h));
         // the solve: tile k of R is (k, 0), of Y is (k, 1); a pair's step
@@ -215,6 +263,60 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
         key = sd.h;
         CK(mu_dag_capture_begin(sd.h, key));
         auto rb = [&](double *M, int k) { return M + (size_t)k * b; };
+        if (prec != MU_PREC_FP64) {
+            // on the FP32 copies, in groups of <= 16 right-hand sides.  The
+            // update of a step is one GEMM over every tile after (forward) or
+            // before (backward) the pair: written as key (a', 0) for the
+            // first tile a' after the pair, and (0, 1) for the rows above it
+            // (every backward step reads (0, 1) as well)
+            auto rhs = [&](auto &&op) -> int {
+                for (int j0 = 0; j0 < nrhs; j0 += 16)
+                    CK(op((size_t)j0 * n, std::min(16, nrhs - j0)));
+                return 0;
+            };
+            for (int q = 0; q < npair; q++) {
+                const int a = 2 * q, w = width(q), last = a + w / b - 1;
+                const int rd0[4] = {a, 0, last, 0};
+                CK(mu_dag_begin_task(sd.h, a, 1, rd0, 2, nullptr));
+                CK(rhs([&](size_t o, int nj) {
+                    return mu::gemm_a32(key, w, nj, w, 1.0, d32(DL32, q), b2, rb(R, a) + o, n,
+                                        0.0, rb(Y, a) + o, n);
+                }));
+                CK(mu_dag_end_task(sd.h));
+                if (last + 1 < t) {
+                    const int rd[4] = {a, 1, last + 1, 0};
+                    CK(mu_dag_begin_task(sd.h, last + 1, 0, rd, 2, nullptr));
+                    CK(rhs([&](size_t o, int nj) {
+                        return mu::gemm_a32(key, (t - last - 1) * b, nj, w, -1.0,
+                                            gt(last + 1, a), n, rb(Y, a) + o, n, 1.0,
+                                            rb(R, last + 1) + o, n);
+                    }));
+                    CK(mu_dag_end_task(sd.h));
+                }
+            }
+            for (int q = npair - 1; q >= 0; q--) {
+                const int a = 2 * q, w = width(q), last = a + w / b - 1;
+                const int rd0[6] = {a, 1, last, 1, 0, 1};
+                CK(mu_dag_begin_task(sd.h, a, 0, rd0, 3, nullptr));
+                CK(rhs([&](size_t o, int nj) {
+                    return mu::gemm_a32(key, w, nj, w, 1.0, d32(DU32, q), b2, rb(Y, a) + o, n,
+                                        0.0, rb(R, a) + o, n);
+                }));
+                CK(mu_dag_end_task(sd.h));
+                if (a > 0) {
+                    const int rd[4] = {a, 0, 0, 1};
+                    CK(mu_dag_begin_task(sd.h, 0, 1, rd, 2, nullptr));
+                    CK(rhs([&](size_t o, int nj) {
+                        return mu::gemm_a32(key, a * b, nj, w, -1.0, gt(0, a), n, rb(R, a) + o,
+                                            n, 1.0, rb(Y, 0) + o, n);
+                    }));
+                    CK(mu_dag_end_task(sd.h));
+                }
+            }
+            CK(mu_dag_capture_end(sd.h, key));
+            CK(mu_exec_create(sd.h, MU_MODE_TILED, nullptr, &sx.h));
+            return 0;
+        }
         for (int q = 0; q < npair; q++) {
             const int a = 2 * q, w = width(q), last = a + w / b - 1;
             const int rd0[4] = {a, 0, last, 0};

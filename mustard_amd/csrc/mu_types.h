@@ -44,6 +44,10 @@ enum Flag : int32_t {
     F_KVEC = 1 << 13,    // GEMM/SYRK with F_SRC_LOW, F_TA, no F_TB: both operands k-contiguous
                          // with 16-byte aligned rows and k a multiple of 16 bytes (the
                          // k_batch_gemm_low kernel)
+    F_A32 = 1 << 14,     // GEMM (op(A) = A, op(B) = B, n <= 16): a stored as FP32, b and c
+                         // FP64; FP32 MFMA on b rounded to FP32, sums added into c in FP64
+                         // (lowered F_NARROW: items of 64 rows x one k-slice of
+                         // NARROW32_KSLICE, gn = number of slices)
 };
 
 // Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).
@@ -58,6 +62,9 @@ constexpr int TRSM_NEST = 256;    // default: triangles above this order nest in
 constexpr int SMALL_MAX = 128;    // largest tile a GETRF/POTRF item factors in LDS
 constexpr int NPRIO = 4;          // priority levels of the device queue
 constexpr int TRINV_BLOCKS = 16;  // diagonal blocks one TRINV item inverts (16 threads each)
+constexpr int NARROW32_KSLICE = 512;  // k per item of a split F_A32 GEMM (a longer k is split
+                                      // over items whose partial sums the last one adds)
+constexpr int NARROW32_SPLIT_BELOW = 128;  // F_A32 GEMMs of fewer 64-row blocks split k
 
 struct UnitDesc {
     int32_t op;
@@ -77,8 +84,13 @@ struct UnitDesc {
     const double *a;
     const double *b;
     double *c;
-    int32_t *info;         // LAPACK-style info cell (atomic min of failing column), or null
-    int32_t ws_slot;       // TRSM with F_INV / TRINV: workspace slot of the inverted blocks, or -1
+    union {
+        int32_t *info;     // LAPACK-style info cell (atomic min of failing column), or null
+        float *part;       // F_A32 GEMM split over gn k-slices: gm x gn partial 64 x 16 tiles,
+                           // then gm arrival counters (zero between launches)
+    };
+    int32_t ws_slot;       // TRSM with F_INV / TRINV: workspace slot of the inverted blocks;
+                           // split F_A32 GEMM: of its partial tiles; or -1
     int32_t grp;           // GEMM: an item covers grp x grp output blocks (0 or 1: one block)
 };
 static_assert(sizeof(UnitDesc) == 128, "UnitDesc must stay 128 bytes");

@@ -114,6 +114,61 @@ def test_dgemm_reduced_precision(torch_gpu, prec, m, n, k, ta, tb, pad):
         assert np.array_equal(got[m:, :], C[m:, :])
 
 
+A32_CASES = [
+    # m, n, k, pad, offset of A (elements), alpha, beta
+    (1, 1, 1, 0, 0, -1.0, 1.0),
+    (64, 16, 32, 0, 0, 1.0, 0.0),
+    (1000, 16, 2000, 0, 0, -1.0, 1.0),   # update of a two-tile solve step
+    (4000, 16, 2000, 0, 0, 1.0, 0.0),    # two-tile diagonal-block inverse times R
+    (8192, 16, 2000, 0, 0, -1.0, 1.0),   # 128 row blocks: k not split
+    (130, 15, 67, 2, 0, 0.5, 2.0),       # ragged rows and k, lda % 4 != 0
+    (101, 3, 37, 0, 1, -1.0, 1.0),       # m % 4 != 0, misaligned A
+    (260, 16, 129, 4, 1, -1.0, 1.0),     # lda % 4 == 0 but A not 16-byte aligned
+    (300, 16, 513, 0, 0, -1.0, 1.0),     # two k-slices, the second one k long
+    (77, 7, 2049, 3, 1, 1.0, 0.0),       # five k-slices, scalar loads
+]
+
+
+@pytest.mark.parametrize("m,n,k,pad,off,alpha,beta", A32_CASES)
+def test_gemm_a32(torch_gpu, m, n, k, pad, off, alpha, beta):
+    """FP32-A narrow GEMM (the refinement solve's substitution step): FP32
+    products of A and B rounded to FP32, summed in FP32 per k-slice of 512
+    (the slices' sums added in FP64) and added into the FP64 C.  Bound per
+    element: an FP32 length-k sum times (|A| |fl32(B)|)_ij, plus the FP64
+    update of C.  Run eagerly: the split's arrival counters start from a
+    freshly zeroed workspace (graph launches that reuse them run in the
+    refinement tests)."""
+    mu = _mu()
+    rng = np.random.default_rng(m * 11 + n * 3 + k + off)
+    lda, ldb, ldc = m + pad, k + pad, m + pad
+    A = rand(rng, lda, k).astype(np.float32)
+    B, C = rand(rng, ldb, n), rand(rng, ldc, n)
+    if beta == 0.0:
+        C[:m, :n] = np.nan  # beta = 0 must not read C
+    flat = np.zeros(off + lda * k, dtype=np.float32)
+    flat[off:] = A.ravel(order="F")
+    dA = torch_gpu.from_numpy(flat).cuda()
+    dB, dC = _dev(B), _dev(C)
+    mu.call("mu_gemm_a32", None, m, n, k, alpha, mu.ptr(dA, off), lda, mu.ptr(dB), ldb, beta,
+            mu.ptr(dC), ldc)
+    torch_gpu.cuda.synchronize()
+    opA = A[:m, :k].astype(np.float64)
+    B32 = B[:k, :n].astype(np.float32).astype(np.float64)
+    ref = alpha * (opA @ B32) + (0.0 if beta == 0.0 else beta * C[:m, :n])
+    got = mu.to_host(dC)
+    tol = (k + 2) * 2.0 ** -24 * abs(alpha) * (np.abs(opA) @ np.abs(B32)) + 2 * EPS * (
+        0.0 if beta == 0.0 else abs(beta) * np.abs(C[:m, :n])) + 1e-300
+    assert np.all(np.abs(got[:m, :n] - ref) <= tol)
+    if pad:
+        assert np.array_equal(got[m:, :], C[m:, :])
+
+
+def test_gemm_a32_rejects_wide_outputs(torch_gpu):
+    mu = _mu()
+    with pytest.raises(Exception, match="n <= 16"):
+        mu.call("mu_gemm_a32", None, 4, 17, 4, 1.0, None, 4, None, 4, 0.0, None, 4)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_dsyrk_reduced_precision(torch_gpu, prec):
     mu = _mu()
