@@ -187,8 +187,10 @@ class Exec:
         self.h = ctypes.c_void_p()
         # the executable points into buffers the DAG recorded; keep it alive
         self._dag = dag
-        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, 1 if lookahead else -1,
-                        sm, 0 if fuse else 1)
+        # lookahead: True / 1 (two lanes), 2 (three lanes: only the chain between
+        # diagonal factorizations on the first), False (one chain)
+        la = (1 if lookahead else -1) if isinstance(lookahead, bool) else int(lookahead)
+        opts = ExecOpts(grid, timeout_s, int(trace), small_max, trsm_max, la, sm, 0 if fuse else 1)
         call("mu_exec_create", dag.h, mode, ctypes.byref(opts), ctypes.byref(self.h))
 
     def launch(self, stream=None):

@@ -609,7 +609,7 @@ int mu_exec_create(mu_dag_t dag, int mode, const mu_exec_opts_t *opts, mu_exec_t
             eo.timeout_s = opts->timeout_s;
         eo.grid = opts->grid;
         eo.trace = opts->trace;
-        eo.lookahead = opts->lookahead >= 0 ? 1 : 0;
+        eo.lookahead = opts->lookahead < 0 ? 0 : opts->lookahead == 2 ? 2 : 1;
         if (opts->sm < 0 || opts->sm > 100)
             return fail(MU_EINVAL, "mu_exec_create: sm is a percentage of the GPU (0 = none)");
         eo.sm = opts->sm;

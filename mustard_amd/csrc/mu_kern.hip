@@ -43,11 +43,12 @@ k_batch_small(const UnitDesc *units, const uint32_t *item_unit, const uint32_t *
 
 // Reduced-precision GEMM/SYRK units whose operands are k-contiguous in
 // their storage precision (the trailing updates of an FP32 / BF16
-// factorization reading the converted panels): long k-slices, 36 KB of LDS.
-__global__ MU_GEMM_KERNEL void k_batch_gemm_low(const UnitDesc *units, const uint32_t *item_unit,
-                                                const uint32_t *items)
+// factorization reading the converted panels): long k-slices; 128 x 128
+// blocks in the level-batched graph (72 KB of LDS, two workgroups per CU).
+__global__ __launch_bounds__(WG_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+k_batch_gemm_low(const UnitDesc *units, const uint32_t *item_unit, const uint32_t *items)
 {
-    __shared__ double smem[LDS_GEMM_LOWK];
+    __shared__ double smem[LDS_GEMM_LOWK128];
     const uint32_t item = items[blockIdx.x];
     const UnitDesc d = units[item_unit[item]];
     run_gemm_lowk_item(d, (int)(item - d.item_base), smem);

@@ -391,7 +391,64 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                 if (d >= 0 && d < (int)sink_task.size())
                     sink_task[d] = 0;
         std::vector<int> lane(nu, 0);
-        if (eopts.lookahead)
+        if (eopts.lookahead == 2) {
+            // Narrow lookahead, three lanes.  Lane 0 carries only the chain
+            // from one diagonal factorization to the next: the factorizations,
+            // the panel solves of the tiles at most two off the diagonal, and
+            // the updates (within two updates) and conversions feeding them;
+            // the other panel solves, with what feeds them as closely, run on
+            // lane 2; every other update on lane 1.  A step then waits for the
+            // solves of the tiles next to the diagonal instead of the whole
+            // panel's, and for the updates of those tiles instead of the whole
+            // previous trailing update.
+            auto rank = [](int l) { return l == 0 ? 0 : l == 2 ? 1 : 2; };
+            auto urgent = [&](int a, int b) { return rank(a) <= rank(b) ? a : b; };
+            for (uint32_t u = 0; u < nu; u++) {
+                if (update_task(u))
+                    continue;
+                const int tk = L.units[u].task;
+                if (tk < 0 || tk >= (int)dag.tasks.size())
+                    continue;
+                if (sink_task[tk]) {
+                    bool inner = true;
+                    for (uint32_t v : L.succ[u])
+                        inner = inner && L.units[v].task == tk;
+                    lane[u] = inner ? 1 : 0;
+                    continue;
+                }
+                const TaskRec &tr = dag.tasks[tk];
+                bool factor = false, solve = false;
+                for (const OpRec &op : tr.ops) {
+                    factor = factor || op.op == OP_GETRF || op.op == OP_POTRF;
+                    solve = solve || op.op == OP_TRSM;
+                }
+                const int dist = std::abs(tr.w.first - tr.w.second);
+                lane[u] = (factor || !solve || dist <= 2) ? 0 : 2;
+            }
+            // updates: the most urgent lane among the solves / factorizations
+            // they feed directly or through one more update
+            std::vector<int> direct(nu, 1);
+            for (uint32_t u = 0; u < nu; u++)
+                if (update_task(u) && L.units[u].op != OP_CVT)
+                    for (uint32_t v : L.succ[u])
+                        if (!update_task(v))
+                            direct[u] = urgent(direct[u], lane[v]);
+            for (uint32_t u = 0; u < nu; u++)
+                if (update_task(u) && L.units[u].op != OP_CVT) {
+                    int l = direct[u];
+                    for (uint32_t v : L.succ[u])
+                        if (update_task(v) && L.units[v].op != OP_CVT)
+                            l = urgent(l, direct[v]);
+                    lane[u] = l;
+                }
+            for (uint32_t u = 0; u < nu; u++)
+                if (update_task(u) && L.units[u].op == OP_CVT) {
+                    int l = 1;
+                    for (uint32_t v : L.succ[u])
+                        l = urgent(l, lane[v]);
+                    lane[u] = l;
+                }
+        } else if (eopts.lookahead)
             for (uint32_t u = 0; u < nu; u++) {
                 const int tk = L.units[u].task;
                 if (update_task(u)) {
@@ -410,7 +467,7 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         // next panel's updates too: it stays on the panel lane unless every
         // update that reads it is bulk work (unit order is topological, so
         // its successors' lanes are settled)
-        if (eopts.lookahead)
+        if (eopts.lookahead == 1)
             for (uint32_t u = 0; u < nu; u++)
                 if (L.units[u].op == OP_CVT && lane[u] == 1)
                     for (uint32_t v : L.succ[u])
@@ -442,12 +499,12 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         // one kernel node per (level, lane, kernel) with work, in that order
         // (a topological order of the node graph).  The nodes of a lane form a
         // chain of levels: each depends on every node of the lane's previous
-        // level; a unit whose predecessor sits in the other lane adds an edge
+        // level; a unit whose predecessor sits in another lane adds an edge
         // between the two nodes.
-        const size_t NK = LEVEL_KINDS;
-        const size_t nkey = 2 * NK * (size_t)std::max(nlev, 1);
+        const size_t NK = LEVEL_KINDS, NL = 3;
+        const size_t nkey = NL * NK * (size_t)std::max(nlev, 1);
         auto ukey = [&](uint32_t u) {
-            return (2 * (size_t)level[u] + lane[u]) * NK + ukind(L.units[u]);
+            return (NL * (size_t)level[u] + lane[u]) * NK + ukind(L.units[u]);
         };
         std::vector<int> node_of(nkey, -1);
         std::vector<std::vector<uint32_t>> lv;
@@ -466,13 +523,22 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                     std::vector<uint32_t> us = by[k];
                     if (eopts.fuse && (k % NK == LEVEL_GEMM || k % NK == LEVEL_GEMM_LOW))
                         us = merge_node(L.units, us, mo);
+                    // the reduced-precision updates run on 128 x 128 blocks here
+                    if (k % NK == LEVEL_GEMM_LOW)
+                        for (uint32_t &u : us) {
+                            UnitDesc d = e->gunits[u];
+                            d.flags |= F_BLK128;
+                            d.gm = (d.m + 127) / 128;
+                            d.gn = d.op == OP_SYRK ? d.gm : (d.n + 127) / 128;
+                            u = mo.add(d);
+                        }
                     std::vector<uint32_t> its;
                     for (uint32_t u : us)
                         for (int i = 0; i < e->gunits[u].nitems; i++)
                             its.push_back(e->gunits[u].item_base + i);
                     node_of[k] = (int)lv.size();
-                    node_lane.push_back((int)((k / NK) & 1));
-                    node_level.push_back((int)(k / NK / 2));
+                    node_lane.push_back((int)((k / NK) % NL));
+                    node_level.push_back((int)(k / NK / NL));
                     e->level_gemm.push_back((char)(k % NK));
                     lv.push_back(std::move(its));
                 }
@@ -484,8 +550,8 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
         std::vector<std::vector<int>> node_deps(nnode);
         {
             // per lane: the nodes of the last level seen, and of the level before
-            std::vector<int> prev[2], cur[2];
-            int cur_level[2] = {-1, -1};
+            std::vector<int> prev[NL], cur[NL];
+            int cur_level[NL] = {-1, -1, -1};
             for (int k = 0; k < nnode; k++) {
                 const int ln = node_lane[k];
                 if (node_level[k] != cur_level[ln]) {
