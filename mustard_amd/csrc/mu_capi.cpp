@@ -205,13 +205,16 @@ int mu::gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A
 }
 
 int mu::cvt_f32(void *stream, const double *src, int lds, float *dst, int ldd, int m, int n,
-                bool transposed)
+                bool transposed, float *plain, int ldp)
 {
-    if (m < 0 || n < 0 || lds < std::max(1, m) || ldd < std::max(1, transposed ? n : m))
+    if (m < 0 || n < 0 || lds < std::max(1, m) || ldd < std::max(1, transposed ? n : m) ||
+        (plain && (!transposed || ldp < std::max(1, m))))
         return fail(MU_EINVAL, "cvt_f32: bad sizes");
     mu::OpRec op;
     op.op = mu::OP_CVT;
-    op.flags = mu::F_LOW_F32 | (transposed ? mu::F_CVT_T : 0);
+    op.flags = mu::F_LOW_F32 | (transposed ? mu::F_CVT_T : 0) | (plain ? mu::F_CVT_BOTH : 0);
+    op.a = reinterpret_cast<const double *>(plain);
+    op.lda = plain ? ldp : 0;
     op.m = m;
     op.n = n;
     op.c = const_cast<double *>(src);

@@ -26,9 +26,10 @@ int gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, in
              const double *B, int ldb, double beta, double *C, int ldc);
 
 // The FP64 m x n block (src, lds) rounded to FP32 into (dst, ldd); with
-// `transposed`, element (r, c) goes to dst[c + r*ldd].  Recorded or run like
-// gemm_a32.
+// `transposed`, element (r, c) goes to dst[c + r*ldd], and with `plain` as
+// well the untransposed copy to (plain, ldp), from the same read.  Recorded
+// or run like gemm_a32.
 int cvt_f32(void *stream, const double *src, int lds, float *dst, int ldd, int m, int n,
-            bool transposed);
+            bool transposed, float *plain = nullptr, int ldp = 0);
 
 }  // namespace mu

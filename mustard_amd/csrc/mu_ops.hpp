@@ -2023,11 +2023,17 @@ __device__ __forceinline__ void cvt_block_t(const UnitDesc &u, int i0, int j0, d
     T *tl = reinterpret_cast<T *>(smem) + w * 16 * 65;  // [column][row], pitch 65
     const MU_G double *src = gp(u.c);
     const int r = i0 + lane;
+    const bool both = (u.flags & F_CVT_BOTH) != 0;
+    MU_G T *plain = (MU_G T *)u.a;
 #pragma unroll
     for (int c = 0; c < 16; c++) {
         const int cc = j0 + 16 * w + c;
-        const double v = (r < u.m && cc < u.n) ? src[r + (size_t)cc * u.ldc] : 0.0;
-        tl[c * 65 + lane] = LowPrec<sizeof(T) == 2 ? 2 : 1>::cv(v);
+        const bool in = r < u.m && cc < u.n;
+        const double v = in ? src[r + (size_t)cc * u.ldc] : 0.0;
+        const T cvv = LowPrec<sizeof(T) == 2 ? 2 : 1>::cv(v);
+        tl[c * 65 + lane] = cvv;
+        if (both && in)
+            plain[r + (size_t)cc * u.lda] = cvv;
     }
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     MU_G T *dst = (MU_G T *)u.b;

@@ -100,6 +100,9 @@ int batch_residual(int nprob, const int *ns, const long long *a_off, const long 
 // inf_norm_work_doubles(m, n) doubles
 size_t inf_norm_work_doubles(int m, int n);
 int inf_norm(const double *A, int lda, int m, int n, double *out, double *work, hipStream_t s);
+// inf_norm of A that also copies A to D (ldd) in the same pass
+int copy_inf_norm(const double *A, int lda, double *D, int ldd, int m, int n, double *out,
+                  double *work, hipStream_t s);
 // X += D
 int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipStream_t s);
 // R = B - A X (A m x n, X n x nrhs) through a K-split partial buffer P of

@@ -233,18 +233,21 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
                         continue;
                     const int rd[2] = {i, j};
                     CK(mu_dag_begin_task(fd.h, i, 2 * t + 6 + j, rd, 1, nullptr));
-                    CK(mu::cvt_f32(key, fb(i, j), n, gt(i, j), n, b, b, false));
-                    if (!lu)  // L^T above the diagonal
-                        CK(mu::cvt_f32(key, fb(i, j), n, gt(j, i), n, b, b, true));
+                    if (lu)
+                        CK(mu::cvt_f32(key, fb(i, j), n, gt(i, j), n, b, b, false));
+                    else  // L, and L^T above the diagonal, from one read
+                        CK(mu::cvt_f32(key, fb(i, j), n, gt(j, i), n, b, b, true, gt(i, j), n));
                     CK(mu_dag_end_task(fd.h));
                 }
             for (int q = 0; q < npair; q++) {
                 const int a = 2 * q, c = a + 1, nr = c < t ? 3 : 1;
                 const int rl[6] = {a, t, q, t + 4, c, t};
                 CK(mu_dag_begin_task(fd.h, q, 3 * t + 6, rl, nr, nullptr));
-                CK(mu::cvt_f32(key, dpair(DL, q, 0, 0), b2, d32(DL32, q), b2, b2, b2, false));
-                if (!lu)
-                    CK(mu::cvt_f32(key, dpair(DL, q, 0, 0), b2, d32(DU32, q), b2, b2, b2, true));
+                if (lu)
+                    CK(mu::cvt_f32(key, dpair(DL, q, 0, 0), b2, d32(DL32, q), b2, b2, b2, false));
+                else
+                    CK(mu::cvt_f32(key, dpair(DL, q, 0, 0), b2, d32(DU32, q), b2, b2, b2, true,
+                                   d32(DL32, q), b2));
                 CK(mu_dag_end_task(fd.h));
                 if (lu) {
                     const int ru[6] = {a, t + 1, q, t + 5, c, t + 1};
@@ -353,8 +356,10 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
         CK(mu_exec_create(sd.h, MU_MODE_TILED, nullptr, &sx.h));
         return 0;
     };
-    auto factor = [&](Ex &fx) -> int {
-        KK(mu::copy_matrix(A, lda, F, n, n, n, s));
+    // F = A (unless the norm's pass already copied it), then factor
+    auto factor = [&](Ex &fx, bool copied) -> int {
+        if (!copied)
+            KK(mu::copy_matrix(A, lda, F, n, n, n, s));
         CK(mu_info_reset(info, s));
         KK(mu::identity_tiles(DL, b2, npair * ninv, s));
         CK(mu_exec_launch(fx.h, s));
@@ -385,7 +390,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     HK(hipEventCreate(&e1.e));
     HK(hipEventCreate(&e2.e));
     HK(hipEventRecord(e0.e, s));
-    KK(mu::inf_norm(A, lda, n, n, an, (double *)dP.p, s));
+    KK(mu::copy_inf_norm(A, lda, F, n, n, n, an, (double *)dP.p, s));
     KK(mu::colmax_abs(B, ldb, n, nrhs, rn, s));
     HK(hipMemcpyAsync(&res->anorm, an, sizeof(double), hipMemcpyDeviceToHost, s));
     HK(hipMemcpyAsync(hb.data(), rn, nrhs * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -425,7 +430,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     }
     bool low_failed = false;
     if (!out_of_range) {
-        CK(factor(fx));
+        CK(factor(fx, true));
         // dsgesv/dsposv (ITER = -3): a reduced-precision factorization that
         // breaks down (a zero pivot, or a Cholesky pivot the rounded updates
         // drove non-positive) goes to the FP64 path when a fallback is allowed
@@ -469,7 +474,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
         Dag fd2, sd2;
         Ex fx2, sx2;
         CK(build(MU_PREC_FP64, fd2, fx2, sd2, sx2));
-        CK(factor(fx2));
+        CK(factor(fx2, false));
         if (res->info != 0) {
             mu_solve_ir_impl_set_error("mu_solve_ir: the FP64 factorization failed (info > 0)");
             return MU_ERUN;

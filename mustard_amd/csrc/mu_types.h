@@ -44,12 +44,13 @@ enum Flag : int32_t {
     F_KVEC = 1 << 13,    // GEMM/SYRK with F_SRC_LOW, F_TA, no F_TB: both operands k-contiguous
                          // with 16-byte aligned rows and k a multiple of 16 bytes (the
                          // k_batch_gemm_low kernel)
-    F_BLK128 = 1 << 15,  // F_KVEC GEMM/SYRK of the level-batched graph: items are 128 x 128
-                         // output blocks (gm, gn count 128-blocks)
     F_A32 = 1 << 14,     // GEMM (op(A) = A, op(B) = B, n <= 16): a stored as FP32, b and c
                          // FP64; FP32 MFMA on b rounded to FP32, sums added into c in FP64
                          // (lowered F_NARROW: items of 64 rows x one k-slice of
                          // NARROW32_KSLICE, gn = number of slices)
+    F_BLK128 = 1 << 15,  // F_KVEC GEMM/SYRK of the level-batched graph: items are 128 x 128
+                         // output blocks (gm, gn count 128-blocks)
+    F_CVT_BOTH = 1 << 16, // CVT with F_CVT_T: also stores the block untransposed at (a, lda)
 };
 
 // Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).

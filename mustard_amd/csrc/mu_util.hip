@@ -350,6 +350,52 @@ __global__ void k_rowsum_max(const double *W, int m, int nchunk, unsigned long l
 
 size_t inf_norm_work_doubles(int m, int n) { return (size_t)((n + INF_KC - 1) / INF_KC) * m; }
 
+// k_rowsum_partial that also copies the chunk it reads to D (the copy of the
+// input a factorization overwrites): one pass over A for both.
+__global__ void k_copy_rowsum_partial(const double *A, size_t lda, double *D, size_t ldd, int m,
+                                      int n, double *W)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j0 = blockIdx.y * INF_KC, j1 = min(n, j0 + INF_KC);
+    if (i >= m)
+        return;
+    const double *a = A + i;
+    double *d = D + i;
+    double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int j = j0;
+    for (; j + 8 <= j1; j += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            v[u] = a[(size_t)(j + u) * lda];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            d[(size_t)(j + u) * ldd] = v[u];
+            s[u] += fabs(v[u]);
+        }
+    }
+    for (; j < j1; j++) {
+        const double v = a[(size_t)j * lda];
+        d[(size_t)j * ldd] = v;
+        s[0] += fabs(v);
+    }
+    W[(size_t)blockIdx.y * m + i] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+}
+
+int copy_inf_norm(const double *A, int lda, double *D, int ldd, int m, int n, double *out,
+                  double *work, hipStream_t s)
+{
+    unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(256), 0, s, o, 1, 0ull);
+    if (m <= 0 || n <= 0)
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    const int nchunk = (n + INF_KC - 1) / INF_KC;
+    hipLaunchKernelGGL(k_copy_rowsum_partial, dim3((m + 255) / 256, nchunk), dim3(256), 0, s, A,
+                       (size_t)lda, D, (size_t)ldd, m, n, work);
+    hipLaunchKernelGGL(k_rowsum_max, dim3((m + 255) / 256), dim3(256), 0, s, work, m, nchunk, o);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int inf_norm(const double *A, int lda, int m, int n, double *out, double *work, hipStream_t s)
 {
     unsigned long long *o = reinterpret_cast<unsigned long long *>(out);
@@ -406,29 +452,37 @@ __global__ __launch_bounds__(256) void k_residual_partial(const double *A, size_
 #pragma unroll
     for (int j = 0; j < 16; j++)
         acc[j] = 0.0;
+    // Every load is issued unconditionally from an address clamped into the
+    // operand (values outside it are zeroed where they are used), so no
+    // branch separates a load from the others in flight: the X staging of a
+    // step is one round trip, and so is each group of 8 A values.
     for (int ks = k0; ks < k1; ks += RES_KS) {
         // stage X[ks .. ks + RES_KS)[0 .. 16) (zero past the chunk / nrhs)
+        double xv[RES_KS * 16 / 256];
 #pragma unroll
         for (int q = 0; q < RES_KS * 16 / 256; q++) {
             const int idx = tid + 256 * q, kq = idx >> 4, j = idx & 15;
-            const int kk = ks + kq;
-            xs[idx] = (kk < k1 && j < nrhs) ? X[(size_t)kk + (size_t)j * ldx] : 0.0;
+            xv[q] = X[(size_t)min(ks + kq, k1 - 1) + (size_t)min(j, nrhs - 1) * ldx];
+        }
+#pragma unroll
+        for (int q = 0; q < RES_KS * 16 / 256; q++) {
+            const int idx = tid + 256 * q, kq = idx >> 4, j = idx & 15;
+            xs[idx] = (ks + kq < k1 && j < nrhs) ? xv[q] : 0.0;
         }
         lds_barrier();
         const int kn = min(RES_KS, k1 - ks);
         for (int kq = 0; kq < kn; kq += 8) {
             double av[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int kk = ks + kq + u;
-                av[u] = (rok && kq + u < kn) ? a[(size_t)min(kk, n - 1) * lda] : 0.0;
-            }
+            for (int u = 0; u < 8; u++)
+                av[u] = a[(size_t)min(ks + kq + u, n - 1) * lda];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
+                const double x = (rok && kq + u < kn) ? av[u] : 0.0;
                 const double *xr = xs + (kq + u) * 16;
 #pragma unroll
                 for (int j = 0; j < 16; j++)
-                    acc[j] = fma(av[u], xr[j], acc[j]);
+                    acc[j] = fma(x, xr[j], acc[j]);
             }
         }
         lds_barrier();
