@@ -135,7 +135,8 @@ int mu_dtrsm(void *stream, char side, char uplo, char transa, char diag, int m, 
  * ones rejected with MU_EINVAL.  Calls that overlap on one GPU share that
  * array and race; a stream being recorded into a DAG is refused
  * (MU_ESTATE).  Allocates no device memory. */
-#define MU_NOALLOC_SCRATCH_BYTES (1u << 20)
+#define MU_NOALLOC_SCRATCH_BYTES (1u << 21)
+#define MU_NOALLOC_MAX_ORDER 8192
 int mu_dtrsm_noalloc(void *stream, char side, char uplo, char transa, char diag, int m, int n,
                      double alpha, const double *A, int lda, double *B, int ldb);
 /* No counterpart in the reference: the substitution step of mu_solve_ir's
@@ -293,14 +294,16 @@ int mu_batch_residual(int nprob, const int *ns, const long long *a_off, const lo
  * No counterpart in the reference: the precision form of the tiled
  * factorizations.  Solves A X = B (A n x n, B and X n x nrhs, device
  * pointers; A and B are not modified) by a tiled factorization whose trailing
- * GEMM/SYRK run in `prec` (diagonal factorizations, panel solves, triangular
- * solves, residual and correction in FP64), followed by the refinement of
- * LAPACK dsgesv/dsposv: x = x + solve(b - A x) until
+ * GEMM/SYRK run in `prec` (diagonal factorizations and panel solves in
+ * FP64), followed by the refinement of LAPACK dsgesv/dsposv: x = x +
+ * solve(b - A x), residual and correction in FP64, the triangular solves on
+ * FP32 copies of the factors when `prec` is FP32 or BF16 (FP64 otherwise), until
  * ||r_j||_inf <= ||x_j||_inf ||A||_inf 2^-53 sqrt(n) for every column, at
  * most itermax times (0: 30); with `fallback`, a run whose reduced-precision
  * factorization breaks down (info > 0; iters is then 0) or whose refinement
- * does not converge is refactored and solved in FP64.  The factors live in a library-owned
- * n x n workspace.  Graph construction happens before the timed region. */
+ * does not converge is refactored and solved in FP64.  The factors live in a
+ * library-owned n x n workspace (plus an n x n FP32 copy for reduced
+ * precision).  Graph construction happens before the timed region. */
 typedef struct {
     int kind;     /* MU_SOLVE_LU or MU_SOLVE_CHOL */
     int tiles;    /* tiles per dimension (n % tiles == 0) */

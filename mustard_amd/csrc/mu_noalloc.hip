@@ -70,6 +70,11 @@ extern "C" int mu_dtrsm_noalloc(void *stream, char side, char uplo, char transa,
         return mu::set_error(MU_ESTATE, "mu_dtrsm_noalloc: the stream is being recorded into a "
                                         "DAG (record with mu_dtrsm; the executable owns its "
                                         "workspace)");
+    if (op.m > MU_NOALLOC_MAX_ORDER)
+        return mu::set_error(MU_EINVAL, "mu_dtrsm_noalloc: a triangle of order " +
+                                            std::to_string(op.m) + " is above the " +
+                                            std::to_string(MU_NOALLOC_MAX_ORDER) +
+                                            " the scratch array is sized for; use mu_dtrsm");
     mu::Dag d;
     d.begin_task({0, 0}, {});
     d.add_op(op);

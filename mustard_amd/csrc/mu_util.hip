@@ -351,7 +351,10 @@ __global__ void k_rowsum_max(const double *W, int m, int nchunk, unsigned long l
 size_t inf_norm_work_doubles(int m, int n) { return (size_t)((n + INF_KC - 1) / INF_KC) * m; }
 
 // k_rowsum_partial that also copies the chunk it reads to D (the copy of the
-// input a factorization overwrites): one pass over A for both.
+// input a factorization overwrites): one pass over A for both.  Groups of 8
+// columns are loaded one group ahead of the stores (unconditional loads
+// from clamped columns, the stores and sums masked), so a thread keeps a
+// group in flight while it writes the previous one.
 __global__ void k_copy_rowsum_partial(const double *A, size_t lda, double *D, size_t ldd, int m,
                                       int n, double *W)
 {
@@ -362,22 +365,27 @@ __global__ void k_copy_rowsum_partial(const double *A, size_t lda, double *D, si
     const double *a = A + i;
     double *d = D + i;
     double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int j = j0;
-    for (; j + 8 <= j1; j += 8) {
-        double v[8];
+    double v[2][8];
+    auto ld = [&](double (&x)[8], int jj) {
 #pragma unroll
         for (int u = 0; u < 8; u++)
-            v[u] = a[(size_t)(j + u) * lda];
+            x[u] = a[(size_t)min(jj + u, j1 - 1) * lda];
+    };
+    auto put = [&](const double (&x)[8], int jj) {
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            d[(size_t)(j + u) * ldd] = v[u];
-            s[u] += fabs(v[u]);
-        }
-    }
-    for (; j < j1; j++) {
-        const double v = a[(size_t)j * lda];
-        d[(size_t)j * ldd] = v;
-        s[0] += fabs(v);
+        for (int u = 0; u < 8; u++)
+            if (jj + u < j1) {
+                d[(size_t)(jj + u) * ldd] = x[u];
+                s[u] += fabs(x[u]);
+            }
+    };
+    ld(v[0], j0);
+    for (int j = j0; j < j1; j += 16) {
+        ld(v[1], j + 8);
+        put(v[0], j);
+        if (j + 16 < j1)
+            ld(v[0], j + 16);
+        put(v[1], j + 8);
     }
     W[(size_t)blockIdx.y * m + i] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }

@@ -80,7 +80,8 @@ def test_two_calls_in_a_row_match_mu_dtrsm(torch_gpu):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("side,uplo,trans,diag,order,other", [
-    ("L", "U", "T", "N", 8192, 8),    # the largest order the scratch holds
+    ("L", "U", "T", "N", 8192, 8),    # the largest order accepted
+    ("L", "L", "N", "N", 8100, 8),    # the order whose nesting needs the most scratch
     ("R", "L", "N", "N", 100, 40),    # order <= 128: no inverses, no scratch
     ("L", "L", "T", "U", 4000, 33),   # ragged blocks of 250, transposed
 ])
@@ -93,7 +94,7 @@ def test_orders_match_mu_dtrsm(torch_gpu, side, uplo, trans, diag, order, other)
 
 
 def test_refusals_before_any_device_work():
-    """Order 8193 needs more scratch than the array holds; a stream being
+    """Order 8193 is above the order the array is sized for; a stream being
     recorded into a DAG is refused (nothing runs: null pointers are fine)."""
     s = ctypes.c_void_p(0)
     L = mu.lib()
