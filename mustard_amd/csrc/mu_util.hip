@@ -460,37 +460,29 @@ __global__ __launch_bounds__(256) void k_residual_partial(const double *A, size_
 #pragma unroll
     for (int j = 0; j < 16; j++)
         acc[j] = 0.0;
-    // Every load is issued unconditionally from an address clamped into the
-    // operand (values outside it are zeroed where they are used), so no
-    // branch separates a load from the others in flight: the X staging of a
-    // step is one round trip, and so is each group of 8 A values.
     for (int ks = k0; ks < k1; ks += RES_KS) {
         // stage X[ks .. ks + RES_KS)[0 .. 16) (zero past the chunk / nrhs)
-        double xv[RES_KS * 16 / 256];
 #pragma unroll
         for (int q = 0; q < RES_KS * 16 / 256; q++) {
             const int idx = tid + 256 * q, kq = idx >> 4, j = idx & 15;
-            xv[q] = X[(size_t)min(ks + kq, k1 - 1) + (size_t)min(j, nrhs - 1) * ldx];
-        }
-#pragma unroll
-        for (int q = 0; q < RES_KS * 16 / 256; q++) {
-            const int idx = tid + 256 * q, kq = idx >> 4, j = idx & 15;
-            xs[idx] = (ks + kq < k1 && j < nrhs) ? xv[q] : 0.0;
+            const int kk = ks + kq;
+            xs[idx] = (kk < k1 && j < nrhs) ? X[(size_t)kk + (size_t)j * ldx] : 0.0;
         }
         lds_barrier();
         const int kn = min(RES_KS, k1 - ks);
         for (int kq = 0; kq < kn; kq += 8) {
             double av[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                av[u] = a[(size_t)min(ks + kq + u, n - 1) * lda];
+            for (int u = 0; u < 8; u++) {
+                const int kk = ks + kq + u;
+                av[u] = (rok && kq + u < kn) ? a[(size_t)min(kk, n - 1) * lda] : 0.0;
+            }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const double x = (rok && kq + u < kn) ? av[u] : 0.0;
                 const double *xr = xs + (kq + u) * 16;
 #pragma unroll
                 for (int j = 0; j < 16; j++)
-                    acc[j] = fma(x, xr[j], acc[j]);
+                    acc[j] = fma(av[u], xr[j], acc[j]);
             }
         }
         lds_barrier();
