@@ -1137,163 +1137,6 @@ __device__ __forceinline__ void report_info(const UnitDesc &u, int col)
                                __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The tile is held in registers: thread t owns rows (t % 16) + 16a and
-// columns (t / 16) + 16b, a, b = 0..7 (a cyclic 8 x 8 share of a 128 x 128
-// tile).  Each column step publishes the pivot row and the scaled pivot column
-// through LDS (double-buffered by step parity, two barriers per step) and
-// every thread applies the rank-1 update to its 64 elements.  Entries of the
-// published row/column at or before the pivot are zero, so the update leaves
-// finished rows and columns unchanged without any per-element test.
-struct SmallTile {
-    double r[8][8];
-};
-
-__device__ __forceinline__ void small_load(SmallTile &T, const MU_G double *a, size_t lda, int n,
-                                           int ti, int tj, bool lower_only)
-{
-#pragma unroll
-    for (int x = 0; x < 8; x++)
-#pragma unroll
-        for (int y = 0; y < 8; y++) {
-            const int row = ti + 16 * x, col = tj + 16 * y;
-            T.r[x][y] = (row < n && col < n && (!lower_only || row >= col))
-                            ? a[row + col * lda]
-                            : 0.0;
-        }
-}
-
-__device__ __forceinline__ void small_store(const SmallTile &T, MU_G double *a, size_t lda, int n,
-                                            int ti, int tj, bool lower_only)
-{
-#pragma unroll
-    for (int x = 0; x < 8; x++)
-#pragma unroll
-        for (int y = 0; y < 8; y++) {
-            const int row = ti + 16 * x, col = tj + 16 * y;
-            if (row < n && col < n && (!lower_only || row >= col))
-                a[row + col * lda] = T.r[x][y];
-        }
-}
-
-// The 16 owners of column c (tj == c % 16) are 16 consecutive lanes of one
-// wave, and the owner of the diagonal entry (ti == c % 16) is one of them, so
-// they take the pivot from it by a lane shuffle: no LDS hand-off and no
-// barrier before the column is scaled.  The pivot row (row owners) and the
-// scaled column (column owners) are then published together, and one barrier
-// per column separates publication from the rank-1 update.
-__device__ __forceinline__ double col_pivot(const SmallTile &T, int blk, int own)
-{
-    double v = 0.0;
-#pragma unroll
-    for (int x = 0; x < 8; x++)
-        if (x == blk)
-            v = T.r[x][x];
-    const int lane = threadIdx.x & 63;
-    return __shfl(v, (lane & ~15) | own, 64);
-}
-
-__device__ __attribute__((always_inline)) inline void getrf_small(const UnitDesc &u, double *smem)
-{
-    const int n = u.m, tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-    MU_G double *a = gp(u.c);
-    const size_t lda = u.ldc;
-    double *rowbuf = smem;        // [2][128]
-    double *colbuf = smem + 256;  // [2][128]
-    SmallTile T;
-    small_load(T, a, lda, n, ti, tj, false);
-    for (int c = 0; c < n; c++) {
-        const int p = c & 1, own = c & 15, blk = c >> 4;
-        if (ti == own) {
-            // owners of row c publish its entries right of the pivot
-#pragma unroll
-            for (int x = 0; x < 8; x++)
-                if (x == blk)
-#pragma unroll
-                    for (int y = 0; y < 8; y++) {
-                        const int col = tj + 16 * y;
-                        rowbuf[p * 128 + col] = col > c ? T.r[x][y] : 0.0;
-                    }
-        }
-        if (tj == own) {
-            // owners of column c scale it below the pivot (by the reciprocal,
-            // as LAPACK's dgetf2 does) and publish it
-            const double piv = col_pivot(T, blk, own);
-            if (ti == own && piv == 0.0)
-                report_info(u, c);
-            const double rp = 1.0 / piv;
-#pragma unroll
-            for (int y = 0; y < 8; y++)
-                if (y == blk)
-#pragma unroll
-                    for (int x = 0; x < 8; x++) {
-                        const int row = ti + 16 * x;
-                        if (row > c)
-                            T.r[x][y] = T.r[x][y] * rp;
-                        colbuf[p * 128 + row] = row > c ? T.r[x][y] : 0.0;
-                    }
-        }
-        lds_barrier();
-        double lv[8], uv[8];
-#pragma unroll
-        for (int x = 0; x < 8; x++)
-            lv[x] = colbuf[p * 128 + ti + 16 * x];
-#pragma unroll
-        for (int y = 0; y < 8; y++)
-            uv[y] = rowbuf[p * 128 + tj + 16 * y];
-#pragma unroll
-        for (int x = 0; x < 8; x++)
-#pragma unroll
-            for (int y = 0; y < 8; y++)
-                T.r[x][y] = fma(-lv[x], uv[y], T.r[x][y]);
-    }
-    small_store(T, a, lda, n, ti, tj, false);
-}
-
-__device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc &u, double *smem)
-{
-    const int n = u.m, tid = threadIdx.x, ti = tid & 15, tj = tid >> 4;
-    MU_G double *a = gp(u.c);
-    const size_t lda = u.ldc;
-    double *colbuf = smem + 256;  // [2][128]
-    SmallTile T;
-    small_load(T, a, lda, n, ti, tj, true);
-    for (int c = 0; c < n; c++) {
-        const int p = c & 1, own = c & 15, blk = c >> 4;
-        if (tj == own) {
-            const double v = col_pivot(T, blk, own);
-            if (ti == own && !(v > 0.0))
-                report_info(u, c);
-            const double d = sqrt(v), rd = 1.0 / d;
-#pragma unroll
-            for (int y = 0; y < 8; y++)
-                if (y == blk)
-#pragma unroll
-                    for (int x = 0; x < 8; x++) {
-                        const int row = ti + 16 * x;
-                        if (row > c)
-                            T.r[x][y] = T.r[x][y] * rd;
-                        else if (row == c)
-                            T.r[x][y] = d;
-                        colbuf[p * 128 + row] = row > c ? T.r[x][y] : 0.0;
-                    }
-        }
-        lds_barrier();
-        double lv[8], lw[8];
-#pragma unroll
-        for (int x = 0; x < 8; x++)
-            lv[x] = colbuf[p * 128 + ti + 16 * x];
-#pragma unroll
-        for (int y = 0; y < 8; y++)
-            lw[y] = colbuf[p * 128 + tj + 16 * y];
-#pragma unroll
-        for (int x = 0; x < 8; x++)
-#pragma unroll
-            for (int y = 0; y < 8; y++)
-                T.r[x][y] = fma(-lv[x], lw[y], T.r[x][y]);
-    }
-    small_store(T, a, lda, n, ti, tj, true);
-}
-
 // ---------------------------------------------------------------------------
 // Blocked diagonal factorizations of an n x n tile (n <= SMALL_MAX), right-
 // looking over panels of 16 columns: no-pivot LU (GETRF) and lower Cholesky
@@ -1320,6 +1163,27 @@ __device__ __attribute__((always_inline)) inline void potrf_small(const UnitDesc
 constexpr int BF_LD = 17;                          // LDS row stride of the panel
 constexpr int LDS_SMALL_PANEL = SMALL_MAX * BF_LD;  // 128 x 16 panel (17 KB)
 constexpr int LDS_SMALL_BLK = LDS_SMALL_PANEL + 16 * BF_LD;  // + inv(L11) for LU
+
+// 1/x and 1/sqrt(x) in FP64 from the hardware estimates (v_rcp_f64,
+// v_rsq_f64) and two Newton steps each (r <- r + r (1 - x r), r <- r + r
+// (1/2 - x r^2 / 2)), within an ulp or two of the rounded results: the
+// panel factorizations' per-column reciprocal and square root, which as IEEE
+// division / sqrt sequences were most of a panel's instructions.  A zero
+// pivot gives NaN (not the division's inf) and a negative one NaN: either way
+// the factors carry non-finite values, and the column is reported as info.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    const double h = 0.5 * x;
+    double r = __builtin_amdgcn_rsq(x);
+    r = fma(r, fma(-h * r, r, 0.5), r);
+    return fma(r, fma(-h * r, r, 0.5), r);
+}
 
 __device__ __forceinline__ double readlane_d(double v, int lane)
 {
@@ -1390,7 +1254,7 @@ __device__ __attribute__((always_inline)) inline void factor_small_blk(const Uni
                     if (!CHOL) {
                         if (lane == 0 && piv == 0.0)
                             report_info(u, p0 + c);
-                        const double l = lane > c ? x[c] * (1.0 / piv) : 0.0;
+                        const double l = lane > c ? x[c] * rcp_nr(piv) : 0.0;
                         if (lane > c)
                             x[c] = l;
                         // lane c's own row keeps its values (multiplier 0)
@@ -1400,11 +1264,11 @@ __device__ __attribute__((always_inline)) inline void factor_small_blk(const Uni
                     } else {
                         if (lane == 0 && !(piv > 0.0))
                             report_info(u, p0 + c);
-                        const double d = sqrt(piv);
+                        const double rd = rsq_nr(piv);
                         if (lane > c)
-                            x[c] *= 1.0 / d;
+                            x[c] *= rd;
                         else if (lane == c)
-                            x[c] = d;
+                            x[c] = piv * rd;
                         const double l = lane > c ? x[c] : 0.0;
 #pragma unroll
                         for (int k = c + 1; k < 16; k++)  // L[p0 + k][c] is lane k's
@@ -1430,12 +1294,12 @@ __device__ __attribute__((always_inline)) inline void factor_small_blk(const Uni
                 for (int c = 0; c < 16; c++) {
                     const double *dr = P + (p0 + c) * BF_LD;  // LU: row c of U11
                     if (!CHOL) {
-                        y[c] *= 1.0 / dr[c];
+                        y[c] *= rcp_nr(dr[c]);
 #pragma unroll
                         for (int k = c + 1; k < 16; k++)
                             y[k] = fma(-y[c], dr[k], y[k]);
                     } else {
-                        y[c] *= 1.0 / dr[c];
+                        y[c] *= rcp_nr(dr[c]);
 #pragma unroll
                         for (int k = c + 1; k < 16; k++)  // L11[k][c]
                             y[k] = fma(-y[c], P[(p0 + k) * BF_LD + c], y[k]);
