@@ -143,9 +143,11 @@ int mu_dtrsm_noalloc(void *stream, char side, char uplo, char transa, char diag,
  * reduced-precision refinement solves.  C = beta C + alpha A B with A an
  * m x k FP32 matrix (column-major, lda in elements, op(A) = A), B k x n and
  * C m x n FP64, 0 <= n <= 16: the products run in FP32 on B rounded to FP32
- * and the sums are added into C in FP64.  Recorded or run like mu_dgemm. */
+ * and the sums are added into C in FP64.  B and C are column-major, or with
+ * `rows` != 0 stored by rows (B(k, j) at B[k*ldb + j], C(i, j) at
+ * C[i*ldc + j]).  Recorded or run like mu_dgemm. */
 int mu_gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
-                const double *B, int ldb, double beta, double *C, int ldc);
+                const double *B, int ldb, double beta, double *C, int ldc, int rows);
 /* cusolverDnDgetrf with devIpiv = NULL -- lu_mustard.cu:360-369.  `info_dev`
  * (device int, may be NULL) receives the 1-based column of the first zero
  * pivot through an atomic minimum: initialise it to INT_MAX (mu_info_reset). */

@@ -84,7 +84,7 @@ SIGNATURES = {
     "mu_event_elapsed_ms": (_i, [ctypes.POINTER(ctypes.c_float), _vp, _vp]),
     "mu_mem_info": (_i, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
     "mu_dgemm": (_i, [_vp, _c, _c, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
-    "mu_gemm_a32": (_i, [_vp, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i]),
+    "mu_gemm_a32": (_i, [_vp, _i, _i, _i, _d, _vp, _i, _vp, _i, _d, _vp, _i, _i]),
     "mu_dsyrk": (_i, [_vp, _c, _c, _i, _i, _d, _vp, _i, _d, _vp, _i]),
     "mu_dtrsm": (_i, [_vp, _c, _c, _c, _c, _i, _i, _d, _vp, _i, _vp, _i]),
     "mu_dtrsm_noalloc": (_i, [_vp, _c, _c, _c, _c, _i, _i, _d, _vp, _i, _vp, _i]),

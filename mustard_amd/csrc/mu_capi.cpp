@@ -181,15 +181,16 @@ int mu::dtrsm_op(char side, char uplo, char transa, char diag, int m, int n, dou
 bool mu::stream_recording(void *stream) { return stream_state(stream).dag != nullptr; }
 
 int mu::gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
-                 const double *B, int ldb, double beta, double *C, int ldc)
+                 const double *B, int ldb, double beta, double *C, int ldc, bool rows)
 {
     if (m < 0 || n < 0 || k < 0 || n > 16)
         return fail(MU_EINVAL, "gemm_a32: need m, k >= 0 and 0 <= n <= 16");
-    if (lda < std::max(1, m) || ldb < std::max(1, k) || ldc < std::max(1, m))
+    if (lda < std::max(1, m) || ldb < std::max(1, rows ? n : k) ||
+        ldc < std::max(1, rows ? n : m))
         return fail(MU_EINVAL, "gemm_a32: leading dimension too small");
     mu::OpRec op;
     op.op = mu::OP_GEMM;
-    op.flags = mu::F_A32;
+    op.flags = mu::F_A32 | (rows ? mu::F_BROW : 0);
     op.m = m;
     op.n = n;
     op.k = k;
@@ -378,9 +379,9 @@ int mu_dgemm(void *stream, char transa, char transb, int m, int n, int k, double
 }
 
 int mu_gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
-                const double *B, int ldb, double beta, double *C, int ldc)
+                const double *B, int ldb, double beta, double *C, int ldc, int rows)
 {
-    return mu::gemm_a32(stream, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+    return mu::gemm_a32(stream, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, rows != 0);
 }
 
 int mu_dsyrk(void *stream, char uplo, char trans, int n, int k, double alpha, const double *A,

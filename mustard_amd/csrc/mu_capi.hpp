@@ -20,10 +20,11 @@ bool stream_recording(void *stream);
 // C = beta C + alpha A B with A an m x k FP32 matrix (column-major, lda in
 // elements), B k x n and C m x n FP64, n <= 16: FP32 products on B rounded
 // to FP32, added into C in FP64 (the substitution steps of a refinement
-// solve on FP32 copies of the factors).  Recorded into the DAG capturing
-// `stream`, or run on it now.
+// solve on FP32 copies of the factors).  B and C are column-major, or with
+// `rows` stored by rows (B(k, j) at B[k*ldb + j], C(i, j) at C[i*ldc + j]).
+// Recorded into the DAG capturing `stream`, or run on it now.
 int gemm_a32(void *stream, int m, int n, int k, double alpha, const float *A, int lda,
-             const double *B, int ldb, double beta, double *C, int ldc);
+             const double *B, int ldb, double beta, double *C, int ldc, bool rows = false);
 
 // The FP64 m x n block (src, lds) rounded to FP32 into (dst, ldd); with
 // `transposed`, element (r, c) goes to dst[c + r*ldd], and with `plain` as

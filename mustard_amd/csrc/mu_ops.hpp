@@ -1638,9 +1638,15 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow_a32(const Unit
     const MU_G double *b = gp(u.b);
     const size_t lda = u.lda, ldb = u.ldb;
     const bool cok = li < ncol;
+    // B(k, j) at b[k sk + j sj], C(i, j) at c[i ci + j cj]: columns, or rows
+    // with F_BROW (a k-step of the 16 x 4 B fragment is then 512 contiguous
+    // bytes instead of 16 columns' 32)
+    const bool brow = (u.flags & F_BROW) != 0;
+    const size_t sk = brow ? ldb : 1, sj = brow ? 1 : ldb;
+    const size_t ci = brow ? (size_t)u.ldc : 1, cj = brow ? 1 : (size_t)u.ldc;
     // loads unconditional from clamped addresses, the k-steps past the
     // slice zeroed as they enter the MFMA (as in gemm_narrow)
-    const MU_G double *bc = b + (size_t)min(li, ncol - 1) * ldb;
+    const MU_G double *bc = b + (size_t)min(li, ncol - 1) * sj;
     const int r0 = i0 + 4 * li;  // this lane's rows r0 .. r0 + 3
     // VEC: m % 4 == 0, so the lane's 4 rows are all inside or all outside
     const int r0c = min(r0, m - 4);
@@ -1655,7 +1661,7 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow_a32(const Unit
 #pragma unroll
         for (int q = 0; q < KU; q++) {
             const int kc = min(k + 4 * q + lk, K - 1);
-            gb[q] = bc[kc];
+            gb[q] = bc[(size_t)kc * sk];
             if (VEC) {
                 ga[q] = *(const MU_G v4f *)(a + r0c + (size_t)kc * lda);
             } else {
@@ -1727,7 +1733,7 @@ __device__ __attribute__((always_inline)) inline void gemm_narrow_a32(const Unit
         for (int e = 0; e < 4; e++) {
             const int row = rb + e;
             if (row < m && cok) {
-                MU_G double *cp = c + row + (size_t)li * u.ldc;
+                MU_G double *cp = c + (size_t)row * ci + (size_t)li * cj;
                 *cp = (beta != 0.0 ? beta * *cp : 0.0) + alpha * v[e];
             }
         }

@@ -103,6 +103,9 @@ int inf_norm(const double *A, int lda, int m, int n, double *out, double *work, 
 // inf_norm of A that also copies A to D (ldd) in the same pass
 int copy_inf_norm(const double *A, int lda, double *D, int ldd, int m, int n, double *out,
                   double *work, hipStream_t s);
+// m x n column-major (ld lds) -> stored by rows (D[i*n + j]) and back
+int to_rows(const double *S, int lds, double *D, int m, int n, hipStream_t s);
+int from_rows(const double *S, double *D, int ldd, int m, int n, hipStream_t s);
 // X += D
 int add_matrix(double *X, int ldx, const double *D, int ldd, int m, int n, hipStream_t s);
 // R = B - A X (A m x n, X n x nrhs) through a K-split partial buffer P of

@@ -135,13 +135,18 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     // and DU32 = DU, or DL^T with Cholesky), the way dsgesv / dsposv solve
     // with their single-precision factors
     const size_t g_elems = ((size_t)n * n + 63) / 64 * 64, d_elems = (size_t)npair * b2 * b2;
-    DevBuf dG;
+    // Their solves keep the right-hand sides stored by rows (Rr, and Y):
+    // a 16-column step then reads each row of B as one 128-byte line.
+    DevBuf dG, dRr;
     float *G = nullptr, *DL32 = nullptr, *DU32 = nullptr;
+    double *Rr = nullptr;
     if (opts->prec != MU_PREC_FP64) {
         HK(hipMalloc(&dG.p, (g_elems + 2 * d_elems) * sizeof(float)));
         G = (float *)dG.p;
         DL32 = G + g_elems;
         DU32 = DL32 + d_elems;
+        HK(hipMalloc(&dRr.p, (size_t)n * nrhs * sizeof(double)));
+        Rr = (double *)dRr.p;
     }
 
     // Graphs over F, R, Y: the factorization (with the inverses of its
@@ -272,9 +277,11 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
             // before (backward) the pair: written as key (a', 0) for the
             // first tile a' after the pair, and (0, 1) for the rows above it
             // (every backward step reads (0, 1) as well)
+            // right-hand sides by rows: row i of Rr / Y at i * nrhs
+            auto rr = [&](double *M, int k) { return M + (size_t)k * b * nrhs; };
             auto rhs = [&](auto &&op) -> int {
                 for (int j0 = 0; j0 < nrhs; j0 += 16)
-                    CK(op((size_t)j0 * n, std::min(16, nrhs - j0)));
+                    CK(op((size_t)j0, std::min(16, nrhs - j0)));
                 return 0;
             };
             for (int q = 0; q < npair; q++) {
@@ -282,8 +289,8 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
                 const int rd0[4] = {a, 0, last, 0};
                 CK(mu_dag_begin_task(sd.h, a, 1, rd0, 2, nullptr));
                 CK(rhs([&](size_t o, int nj) {
-                    return mu::gemm_a32(key, w, nj, w, 1.0, d32(DL32, q), b2, rb(R, a) + o, n,
-                                        0.0, rb(Y, a) + o, n);
+                    return mu::gemm_a32(key, w, nj, w, 1.0, d32(DL32, q), b2, rr(Rr, a) + o, nrhs,
+                                        0.0, rr(Y, a) + o, nrhs, true);
                 }));
                 CK(mu_dag_end_task(sd.h));
                 if (last + 1 < t) {
@@ -291,8 +298,8 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
                     CK(mu_dag_begin_task(sd.h, last + 1, 0, rd, 2, nullptr));
                     CK(rhs([&](size_t o, int nj) {
                         return mu::gemm_a32(key, (t - last - 1) * b, nj, w, -1.0,
-                                            gt(last + 1, a), n, rb(Y, a) + o, n, 1.0,
-                                            rb(R, last + 1) + o, n);
+                                            gt(last + 1, a), n, rr(Y, a) + o, nrhs, 1.0,
+                                            rr(Rr, last + 1) + o, nrhs, true);
                     }));
                     CK(mu_dag_end_task(sd.h));
                 }
@@ -302,16 +309,16 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
                 const int rd0[6] = {a, 1, last, 1, 0, 1};
                 CK(mu_dag_begin_task(sd.h, a, 0, rd0, 3, nullptr));
                 CK(rhs([&](size_t o, int nj) {
-                    return mu::gemm_a32(key, w, nj, w, 1.0, d32(DU32, q), b2, rb(Y, a) + o, n,
-                                        0.0, rb(R, a) + o, n);
+                    return mu::gemm_a32(key, w, nj, w, 1.0, d32(DU32, q), b2, rr(Y, a) + o, nrhs,
+                                        0.0, rr(Rr, a) + o, nrhs, true);
                 }));
                 CK(mu_dag_end_task(sd.h));
                 if (a > 0) {
                     const int rd[4] = {a, 0, 0, 1};
                     CK(mu_dag_begin_task(sd.h, 0, 1, rd, 2, nullptr));
                     CK(rhs([&](size_t o, int nj) {
-                        return mu::gemm_a32(key, a * b, nj, w, -1.0, gt(0, a), n, rb(R, a) + o,
-                                            n, 1.0, rb(Y, 0) + o, n);
+                        return mu::gemm_a32(key, a * b, nj, w, -1.0, gt(0, a), n, rr(Rr, a) + o,
+                                            nrhs, 1.0, rr(Y, 0) + o, nrhs, true);
                     }));
                     CK(mu_dag_end_task(sd.h));
                 }
@@ -444,9 +451,19 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
     if (out_of_range || low_failed) {
         res->iters = 0;
     } else {
+        // the solve of R in place (through Rr for the reduced-precision graph)
+        const bool rows = opts->prec != MU_PREC_FP64;
+        auto solve = [&]() -> int {
+            if (rows)
+                KK(mu::to_rows(R, n, Rr, n, nrhs, s));
+            CK(mu_exec_launch(sx.h, s));
+            if (rows)
+                KK(mu::from_rows(Rr, R, n, n, nrhs, s));
+            return 0;
+        };
         // x0 = solve(b)
         KK(mu::copy_matrix(B, ldb, R, n, n, nrhs, s));
-        CK(mu_exec_launch(sx.h, s));
+        CK(solve());
         KK(mu::copy_matrix(R, n, X, ldx, n, nrhs, s));
         int it = 0;
         for (;; it++) {
@@ -464,7 +481,7 @@ extern "C" int mu_solve_ir(const double *A, int n, int lda, const double *B, int
             if (it >= itermax || !std::isfinite(w))
                 break;
             // d = solve(r) in place, x += d
-            CK(mu_exec_launch(sx.h, s));
+            CK(solve());
             KK(mu::add_matrix(X, ldx, R, n, n, nrhs, s));
         }
         res->iters = it;

@@ -51,6 +51,8 @@ enum Flag : int32_t {
     F_BLK128 = 1 << 15,  // F_KVEC GEMM/SYRK of the level-batched graph: items are 128 x 128
                          // output blocks (gm, gn count 128-blocks)
     F_CVT_BOTH = 1 << 16, // CVT with F_CVT_T: also stores the block untransposed at (a, lda)
+    F_BROW = 1 << 17,    // F_A32 GEMM: B and C stored by rows (B(k, j) at b[k*ldb + j], C(i, j)
+                         // at c[i*ldc + j])
 };
 
 // Kernel geometry (fixed for gfx950: 4 waves of 64 lanes per workgroup).

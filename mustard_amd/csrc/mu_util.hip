@@ -417,6 +417,32 @@ int inf_norm(const double *A, int lda, int m, int n, double *out, double *work, 
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Column-major (ld lds) <-> stored by rows (ld n): the m x n right-hand
+// sides of the reduced-precision solves, whose steps read 16 columns of a
+// row as one 128-byte line.
+__global__ void k_to_rows(const double *S, size_t lds, double *D, int m, int n)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)m * n;
+         k += (size_t)gridDim.x * blockDim.x)
+        D[k] = S[(k / n) + (k % n) * lds];
+}
+__global__ void k_from_rows(const double *S, double *D, size_t ldd, int m, int n)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)m * n;
+         k += (size_t)gridDim.x * blockDim.x)
+        D[(k / n) + (k % n) * ldd] = S[k];
+}
+int to_rows(const double *S, int lds, double *D, int m, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_rows, dim3(512), dim3(256), 0, s, S, (size_t)lds, D, m, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+int from_rows(const double *S, double *D, int ldd, int m, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_from_rows, dim3(512), dim3(256), 0, s, S, D, (size_t)ldd, m, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // X += D (m x n)
 __global__ void k_add_matrix(double *X, size_t ldx, const double *D, size_t ldd, int m, int n)
 {

@@ -129,44 +129,57 @@ A32_CASES = [
 ]
 
 
+@pytest.mark.parametrize("rows", [0, 1])
 @pytest.mark.parametrize("m,n,k,pad,off,alpha,beta", A32_CASES)
-def test_gemm_a32(torch_gpu, m, n, k, pad, off, alpha, beta):
+def test_gemm_a32(torch_gpu, m, n, k, pad, off, alpha, beta, rows):
     """FP32-A narrow GEMM (the refinement solve's substitution step): FP32
     products of A and B rounded to FP32, summed in FP32 per k-slice of 512
     (the slices' sums added in FP64) and added into the FP64 C.  Bound per
     element: an FP32 length-k sum times (|A| |fl32(B)|)_ij, plus the FP64
-    update of C.  Run eagerly: the split's arrival counters start from a
-    freshly zeroed workspace (graph launches that reuse them run in the
-    refinement tests)."""
+    update of C.  B and C column-major, or stored by rows.  Run eagerly: the
+    split's arrival counters start from a freshly zeroed workspace (graph
+    launches that reuse them run in the refinement tests)."""
     mu = _mu()
     rng = np.random.default_rng(m * 11 + n * 3 + k + off)
-    lda, ldb, ldc = m + pad, k + pad, m + pad
+    lda = m + pad
     A = rand(rng, lda, k).astype(np.float32)
-    B, C = rand(rng, ldb, n), rand(rng, ldc, n)
+    # B (k x n) and C (m x n) with `pad` spare rows (column-major) or spare
+    # columns (stored by rows: the by-rows matrix is the transpose in memory)
+    B = rand(rng, k + pad, n) if not rows else rand(rng, k, n + pad)
+    C = rand(rng, m + pad, n) if not rows else rand(rng, m, n + pad)
     if beta == 0.0:
         C[:m, :n] = np.nan  # beta = 0 must not read C
     flat = np.zeros(off + lda * k, dtype=np.float32)
     flat[off:] = A.ravel(order="F")
     dA = torch_gpu.from_numpy(flat).cuda()
-    dB, dC = _dev(B), _dev(C)
+    if rows:
+        ldb, ldc = n + pad, n + pad
+        dB = torch_gpu.from_numpy(np.ascontiguousarray(B)).cuda()
+        dC = torch_gpu.from_numpy(np.ascontiguousarray(C)).cuda()
+    else:
+        ldb, ldc = k + pad, m + pad
+        dB, dC = _dev(B), _dev(C)
     mu.call("mu_gemm_a32", None, m, n, k, alpha, mu.ptr(dA, off), lda, mu.ptr(dB), ldb, beta,
-            mu.ptr(dC), ldc)
+            mu.ptr(dC), ldc, rows)
     torch_gpu.cuda.synchronize()
     opA = A[:m, :k].astype(np.float64)
     B32 = B[:k, :n].astype(np.float32).astype(np.float64)
     ref = alpha * (opA @ B32) + (0.0 if beta == 0.0 else beta * C[:m, :n])
-    got = mu.to_host(dC)
+    got = dC.cpu().numpy() if rows else mu.to_host(dC)
     tol = (k + 2) * 2.0 ** -24 * abs(alpha) * (np.abs(opA) @ np.abs(B32)) + 2 * EPS * (
         0.0 if beta == 0.0 else abs(beta) * np.abs(C[:m, :n])) + 1e-300
     assert np.all(np.abs(got[:m, :n] - ref) <= tol)
-    if pad:
-        assert np.array_equal(got[m:, :], C[m:, :])
+    if pad:  # nothing outside the m x n block may change
+        if rows:
+            assert np.array_equal(got[:, n:], C[:, n:])
+        else:
+            assert np.array_equal(got[m:, :], C[m:, :])
 
 
 def test_gemm_a32_rejects_wide_outputs(torch_gpu):
     mu = _mu()
     with pytest.raises(Exception, match="n <= 16"):
-        mu.call("mu_gemm_a32", None, 4, 17, 4, 1.0, None, 4, None, 4, 0.0, None, 4)
+        mu.call("mu_gemm_a32", None, 4, 17, 4, 1.0, None, 4, None, 4, 0.0, None, 4, 0)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
