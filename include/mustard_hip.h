@@ -220,15 +220,15 @@ int mu_exec_level_items(mu_exec_t exec, int level, long long *items);
 int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 /* The kernel one level runs on (MU_KERNEL_GENERIC: k_batch, any item kind;
  * MU_KERNEL_GEMM: k_batch_gemm, GEMM/SYRK blocks only; MU_KERNEL_SMALL:
- * k_batch_small, diagonal GETRF/POTRF of order <= 128 and precision
- * conversions only; MU_KERNEL_TRSM: k_batch_trsm, 8-wave panel solves with
+ * k_batch_small, diagonal GETRF/POTRF of order <= 128 only; MU_KERNEL_TRSM: k_batch_trsm, 8-wave panel solves with
  * inverted diagonal blocks only; MU_KERNEL_TRSM_SMALL: k_batch_trsm_small,
  * panel solves of order <= 128 without inverted blocks only; MU_KERNEL_TRSM_256 /
  * MU_KERNEL_TRSM_512: k_batch_trsm_mid<256 / 512>, 4-wave panel solves with
  * inverted diagonal blocks of order <= 256 / 512 and the block inversions
  * before them; MU_KERNEL_GEMM_NARROW: k_batch_gemm_narrow, GEMMs with at most
  * 16 output columns; MU_KERNEL_GEMM_LOW: k_batch_gemm_low, FP32 / BF16 updates
- * reading the converted panels, both operands k-contiguous) and the algorithmic
+ * reading the converted panels, both operands k-contiguous; MU_KERNEL_CVT:
+ * k_batch_cvt, precision conversions only) and the algorithmic
  * flops of its units (GEMM 2mnk, SYRK n(n+1)k, TRSM m^2 n, GETRF 2m^3/3,
  * POTRF m^3/3).  Either output may be NULL. */
 #define MU_KERNEL_GENERIC 0
@@ -240,6 +240,7 @@ int mu_exec_level_ops(mu_exec_t exec, int level, long long *counts);
 #define MU_KERNEL_TRSM_512 6
 #define MU_KERNEL_GEMM_NARROW 7
 #define MU_KERNEL_GEMM_LOW 8
+#define MU_KERNEL_CVT 9
 int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops);
 /* After the stream is synchronized: MU_ERUN if the scheduler timed out. */
 int mu_exec_check(mu_exec_t exec);

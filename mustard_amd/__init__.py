@@ -216,7 +216,7 @@ class Exec:
         return {1: "k_batch_gemm", 2: "k_batch_small", 3: "k_batch_trsm",
                 4: "k_batch_trsm_small", 5: "k_batch_trsm_256",
                 6: "k_batch_trsm_512", 7: "k_batch_gemm_narrow",
-                8: "k_batch_gemm_low"}.get(k.value, "k_batch"), f.value
+                8: "k_batch_gemm_low", 9: "k_batch_cvt"}.get(k.value, "k_batch"), f.value
 
     def level_items(self, level):
         n = ctypes.c_longlong()

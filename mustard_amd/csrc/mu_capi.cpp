@@ -708,6 +708,7 @@ int mu_exec_level_info(mu_exec_t exec, int level, int *kernel, double *flops)
                   : e->level_gemm[level] == mu::LEVEL_TRSM_512   ? MU_KERNEL_TRSM_512
                   : e->level_gemm[level] == mu::LEVEL_GEMM_NARROW ? MU_KERNEL_GEMM_NARROW
                   : e->level_gemm[level] == mu::LEVEL_GEMM_LOW    ? MU_KERNEL_GEMM_LOW
+                  : e->level_gemm[level] == mu::LEVEL_CVT         ? MU_KERNEL_CVT
                                                                  : MU_KERNEL_GENERIC;
     if (flops)
         *flops = e->level_flops[level];

@@ -54,6 +54,20 @@ k_batch_gemm_low(const UnitDesc *units, const uint32_t *item_unit, const uint32_
     run_gemm_lowk_item(d, (int)(item - d.item_base), smem);
 }
 
+// Precision conversions (the panels' reduced-precision shadows, the FP32
+// copies of the factors): 16 KB of LDS and few registers, so eight
+// workgroups share a CU -- a conversion on the lookahead chain waits for no
+// CU to drain, and the bulk ones stream at full occupancy.
+__global__ __launch_bounds__(WG_THREADS) void k_batch_cvt(const UnitDesc *units,
+                                                          const uint32_t *item_unit,
+                                                          const uint32_t *items)
+{
+    __shared__ double smem[4 * 16 * 65 / 2 + 4];
+    const uint32_t item = items[blockIdx.x];
+    const UnitDesc d = units[item_unit[item]];
+    cvt_block(d, (int)(item - d.item_base), smem);
+}
+
 // GEMMs with at most 16 output columns (substitution steps of a solve).
 __global__ __launch_bounds__(WG_THREADS) void k_batch_gemm_narrow(const UnitDesc *units,
                                                                   const uint32_t *item_unit,
@@ -103,6 +117,7 @@ const void *level_kernel(int kind)
     case LEVEL_TRSM_SMALL: return reinterpret_cast<const void *>(k_batch_trsm_small);
     case LEVEL_GEMM_NARROW: return reinterpret_cast<const void *>(k_batch_gemm_narrow);
     case LEVEL_GEMM_LOW: return reinterpret_cast<const void *>(k_batch_gemm_low);
+    case LEVEL_CVT: return reinterpret_cast<const void *>(k_batch_cvt);
     case LEVEL_TRSM_256: return reinterpret_cast<const void *>(k_batch_trsm_mid<256>);
     case LEVEL_TRSM_512: return reinterpret_cast<const void *>(k_batch_trsm_mid<512>);
     default: return generic_kernel(kind);

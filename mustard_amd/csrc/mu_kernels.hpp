@@ -19,7 +19,8 @@ enum LevelKernel : char {
     LEVEL_TRSM_512 = 6,
     LEVEL_GEMM_NARROW = 7,
     LEVEL_GEMM_LOW = 8,  // reduced-precision GEMM/SYRK with k-contiguous operands (F_KVEC)
-    LEVEL_KINDS = 9,
+    LEVEL_CVT = 9,       // precision conversions (OP_CVT)
+    LEVEL_KINDS = 10,
 };
 
 constexpr int TRSM_KERNEL_WAVES = 8;  // k_batch_trsm: two waves per SIMD

@@ -486,6 +486,8 @@ static int exec_build(Exec *e, const Dag &dag, int mode, const LowerOpts &lopts,
                 return LEVEL_GEMM_LOW;
             if (gemm_only_op(d.op))
                 return LEVEL_GEMM;
+            if (d.op == OP_CVT)
+                return LEVEL_CVT;
             if (small_only_op(d.op))
                 return LEVEL_SMALL;
             if (d.op == OP_TRINV)
