@@ -82,6 +82,27 @@ def test_solve_ir(torch_gpu, oracle, kind, prec, kappa, maxit, n, t):
 
 
 @pytest.mark.parametrize("kind", ["lu", "chol"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_solve_ir_more_than_16_rhs(torch_gpu, oracle, kind, prec):
+    """20 right-hand sides: the reduced-precision solve runs them as groups of
+    16 and 4 columns of the row-stored right-hand sides (leading dimension
+    20); the refinement still reaches the FP64 backward error."""
+    mu = _mu()
+    n, t, nrhs = 2000, 4, 20
+    A = oracle.gen_cond(n, 1e3, 11)
+    B = oracle.gen_uniform((n, nrhs), 12, -1.0, 1.0)
+    dA, dB = mu.to_device(A), mu.to_device(B)
+    dX, res = mu.solve_ir(kind, dA, dB, n, nrhs, t, prec=prec, fallback=False)
+    torch_gpu.cuda.synchronize()
+    X = mu.to_host(dX)
+    print(kind, prec, res)
+    assert res["info"] == 0 and res["converged"] == 1 and res["fell_back"] == 0
+    assert oracle.backward_error(A, X, B) <= 10 * n * EPS
+    ref = oracle.solve_fp64(kind, A, B)
+    assert np.max(np.abs(X - ref)) <= 10 * 1e3 * n * EPS * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("kind", ["lu", "chol"])
 def test_solve_ir_bf16_kappa_1e6(torch_gpu, oracle, kind):
     """BF16 updates at kappa = 1e6 (kappa u_bf16 ~ 4e3): plain refinement is
     not expected to converge.  With the fallback the run must end at the
