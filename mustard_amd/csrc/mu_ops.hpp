@@ -490,149 +490,16 @@ __device__ __attribute__((always_inline)) inline void gemm_block_low(const UnitD
 // ---------------------------------------------------------------------------
 // Reduced-precision GEMM / SYRK block with both operands k-contiguous in
 // their storage precision (F_KVEC: op(A) = A^T of a panel the conversion
-// stored transposed, op(B) = B), C = beta C + alpha op(A) op(B) on a 64 x 64
-// block.  The k-slices are long (BF16: 64, FP32: 32) so a wave runs 16 (BF16)
-// or 32 (FP32) MFMAs between barriers, and every operand row reaches LDS as
-// whole 16-byte vectors: one global load and one ds_write_b128 per 16 bytes.
-// LDS rows are padded by 16 bytes.  Same 2 x 2 waves of 32 x 32 and FP32
-// result layout as gemm_block_low.  Measured on the merged LU bulk update
-// (22000 x 22000 x 1000) in isolation: BF16 208 TFLOP/s, FP32 110.
-// ---------------------------------------------------------------------------
-template <int P>
-__device__ __attribute__((always_inline)) inline void gemm_block_lowk(const UnitDesc &u, int bi,
-                                                                      int bj, bool lower_only,
-                                                                      double *smem)
-{
-    using T = typename LowPrec<P>::T;
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    constexpr int ES = sizeof(T), VE = 16 / ES;        // elements per 16 bytes
-    constexpr int BK = P == 2 ? 64 : 32, PK = BK + VE;  // k-slice, LDS row pitch (elements)
-    constexpr int CPR = BK / VE, NL = 64 * BK / VE / WG_THREADS;  // chunks per row, loads/thread
-    static_assert(NL * WG_THREADS * VE == 64 * BK, "loader split");
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1, li = lane & 15, lk = lane >> 4;
-    const int i0 = bi * GEMM_BM, j0 = bj * GEMM_BN;
-    const int rows = min(GEMM_BM, u.m - i0), cols = min(GEMM_BN, u.n - j0);
-    const int K = u.k;
-    const MU_G T *A = (const MU_G T *)u.a + (size_t)i0 * u.lda;  // op(A) row r at A + r lda
-    const MU_G T *B = (const MU_G T *)u.b + (size_t)j0 * u.ldb;  // op(B) column c at B + c ldb
-    T *SL = reinterpret_cast<T *>(smem);
-    T *As[2] = {SL, SL + 128 * PK};
-    T *Bs[2] = {SL + 64 * PK, SL + 192 * PK};
-
-    v4u ra0[NL], rb0[NL], ra1[NL], rb1[NL];
-    // chunk ch of row r of the k-slice at k0; a chunk past K reads the
-    // slice's last full chunk (or the row's first) and is zeroed when stored
-    auto load = [&](v4u (&xa)[NL], v4u (&xb)[NL], int k0) {
-        const int kc = K - k0, ks = kc > 0 ? k0 : 0;
-#pragma unroll
-        for (int q = 0; q < NL; q++) {
-            const int idx = tid + WG_THREADS * q, r = idx / CPR, ch = idx % CPR;
-            const int kk = min(ch * VE, max(kc - VE, 0));
-            xa[q] = *(const MU_G v4u *)(A + (size_t)min(r, rows - 1) * u.lda + ks + kk);
-            xb[q] = *(const MU_G v4u *)(B + (size_t)min(r, cols - 1) * u.ldb + ks + kk);
-        }
-    };
-    auto store = [&](const v4u (&xa)[NL], const v4u (&xb)[NL], int buf, int k0) {
-        const int kc = K - k0;
-        const v4u zero = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int q = 0; q < NL; q++) {
-            const int idx = tid + WG_THREADS * q, r = idx / CPR, ch = idx % CPR;
-            const bool kin = ch * VE < kc;
-            *(v4u *)(As[buf] + r * PK + ch * VE) = (kin && r < rows) ? xa[q] : zero;
-            *(v4u *)(Bs[buf] + r * PK + ch * VE) = (kin && r < cols) ? xb[q] : zero;
-        }
-    };
-    v4f acc[2][2];
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++)
-            acc[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
-    auto compute = [&](int buf) {
-        const T *Sa = As[buf], *Sb = Bs[buf];
-        if constexpr (P == 2) {
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 16) {
-                v4s fa[2], fb[2];
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-                    fa[x] = *reinterpret_cast<const v4s *>(Sa + (wm * 32 + x * 16 + li) * PK + kk + 4 * lk);
-#pragma unroll
-                for (int y = 0; y < 2; y++)
-                    fb[y] = *reinterpret_cast<const v4s *>(Sb + (wn * 32 + y * 16 + li) * PK + kk + 4 * lk);
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-#pragma unroll
-                    for (int y = 0; y < 2; y++)
-                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(fa[x], fb[y], acc[x][y], 0, 0, 0);
-            }
-        } else {
-#pragma unroll
-            for (int kk = 0; kk < BK; kk += 4) {
-                float fa[2], fb[2];
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-                    fa[x] = Sa[(wm * 32 + x * 16 + li) * PK + kk + lk];
-#pragma unroll
-                for (int y = 0; y < 2; y++)
-                    fb[y] = Sb[(wn * 32 + y * 16 + li) * PK + kk + lk];
-#pragma unroll
-                for (int x = 0; x < 2; x++)
-#pragma unroll
-                    for (int y = 0; y < 2; y++)
-                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[x], fb[y], acc[x][y], 0, 0, 0);
-            }
-        }
-    };
-    const int nkb = (K + BK - 1) / BK;
-    if (K > 0) {
-        load(ra0, rb0, 0);
-        store(ra0, rb0, 0, 0);
-        load(ra1, rb1, BK);
-        lds_barrier();
-        for (int kb = 0; kb + 1 < nkb; kb += 2) {
-            load(ra0, rb0, (kb + 2) * BK);
-            compute(0);
-            store(ra1, rb1, 1, (kb + 1) * BK);
-            lds_barrier();
-            load(ra1, rb1, (kb + 3) * BK);
-            compute(1);
-            store(ra0, rb0, 0, (kb + 2) * BK);
-            lds_barrier();
-        }
-        if (nkb & 1)
-            compute(0);
-    }
-    const double alpha = u.alpha, beta = u.beta;
-#pragma unroll
-    for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int r = wm * 32 + x * 16 + f32_row(e, lk);
-                const int c = wn * 32 + y * 16 + li;
-                if (r < rows && c < cols && (!lower_only || i0 + r >= j0 + c)) {
-                    MU_G double *cp = gp(u.c) + (i0 + r) + (size_t)(j0 + c) * u.ldc;
-                    const double old = beta != 0.0 ? *cp : 0.0;
-                    *cp = beta * old + alpha * (double)acc[x][y][e];
-                }
-            }
-}
-// LDS of gemm_block_lowk: 2 buffers x (64 + 64) rows x (BK + VE) elements = 36 KB
-constexpr int LDS_GEMM_LOWK = 2 * 128 * (64 + 8) * 2 / 8;
-
-// ---------------------------------------------------------------------------
-// The same update on 128 x 128 output blocks (units flagged F_BLK128, made
-// for the level-batched graph): each of the 4 waves owns a 64 x 64 quarter,
-// 4 x 4 MFMA tiles, so every LDS fragment feeds four MFMAs and every byte
-// staged from global memory twice the flops of the 64 x 64 block.  BF16 runs
-// on gfx950's v_mfma_f32_16x16x32_bf16 (8 k-values per lane, lane group lk
-// holding k = 8 lk .. 8 lk + 7 of A's row and B's column), FP32 on
-// v_mfma_f32_16x16x4f32.  The k-slice (BF16 64, FP32 32) moves global ->
-// registers -> LDS one slice ahead of the MFMAs, as in gemm_block_lowk.
+// stored transposed, op(B) = B), C = beta C + alpha op(A) op(B) on a
+// 128 x 128 output block (units flagged F_BLK128, made for the level-batched
+// graph).  Each of the 4 waves owns a 64 x 64 quarter, 4 x 4 MFMA tiles, so
+// every LDS fragment feeds four MFMAs.  BF16 runs on gfx950's
+// v_mfma_f32_16x16x32_bf16 (8 k-values per lane, lane group lk holding
+// k = 8 lk .. 8 lk + 7 of A's row and B's column), FP32 on
+// v_mfma_f32_16x16x4f32; FP32 accumulation, the result applied to the FP64
+// C.  The k-slices are long (BF16: 64, FP32: 32) and every operand row
+// reaches LDS as whole 16-byte vectors (LDS rows padded by 16 bytes),
+// global -> registers -> LDS one slice ahead of the MFMAs.
 // ---------------------------------------------------------------------------
 template <int P>
 __device__ __attribute__((always_inline)) inline void gemm_block_lowk128(const UnitDesc &u,
@@ -1847,25 +1714,12 @@ __device__ __attribute__((always_inline)) inline void run_gemm_item(const UnitDe
     }
 }
 
-// GEMM or SYRK item of a unit flagged F_KVEC (k_batch_gemm_low).
+// GEMM or SYRK item of a unit flagged F_KVEC | F_BLK128 (k_batch_gemm_low;
+// the graph builder gives every unit of that kernel 128 x 128 blocks).
 __device__ __attribute__((always_inline)) inline void run_gemm_lowk_item(const UnitDesc &u, int it,
                                                                          double *smem)
 {
     int bi, bj;
-    if (u.flags & F_BLK128) {
-        bool lo = false;
-        if (u.op == OP_SYRK) {
-            syrk_block(u.gm, it, bi, bj);
-            lo = bi == bj;
-        } else {
-            gemm_block_order(u.gm, u.gn, it, bi, bj);
-        }
-        if (u.flags & F_LOW_BF16)
-            gemm_block_lowk128<2>(u, bi, bj, lo, smem);
-        else
-            gemm_block_lowk128<1>(u, bi, bj, lo, smem);
-        return;
-    }
     bool lo = false;
     if (u.op == OP_SYRK) {
         syrk_block(u.gm, it, bi, bj);
@@ -1874,9 +1728,9 @@ __device__ __attribute__((always_inline)) inline void run_gemm_lowk_item(const U
         gemm_block_order(u.gm, u.gn, it, bi, bj);
     }
     if (u.flags & F_LOW_BF16)
-        gemm_block_lowk<2>(u, bi, bj, lo, smem);
+        gemm_block_lowk128<2>(u, bi, bj, lo, smem);
     else
-        gemm_block_lowk<1>(u, bi, bj, lo, smem);
+        gemm_block_lowk128<1>(u, bi, bj, lo, smem);
 }
 
 // FP64 block -> FP32 / BF16 storage: item = one 64 x 64 block of the m x n
