@@ -1029,7 +1029,8 @@ __device__ __forceinline__ void report_info(const UnitDesc &u, int col)
 // ---------------------------------------------------------------------------
 constexpr int BF_LD = 17;                          // LDS row stride of the panel
 constexpr int LDS_SMALL_PANEL = SMALL_MAX * BF_LD;  // 128 x 16 panel (17 KB)
-constexpr int LDS_SMALL_BLK = LDS_SMALL_PANEL + 16 * BF_LD;  // + inv(L11) for LU
+// + inv(L11) for LU, + 1 / diag(L11) for Cholesky
+constexpr int LDS_SMALL_BLK = LDS_SMALL_PANEL + 16 * BF_LD + 16;
 
 // 1/x and 1/sqrt(x) in FP64 from the hardware estimates (v_rcp_f64,
 // v_rsq_f64) and two Newton steps each (r <- r + r (1 - x r), r <- r + r
@@ -1074,6 +1075,7 @@ __device__ __attribute__((always_inline)) inline void factor_small_blk(const Uni
     const size_t lda = u.ldc;
     double *P = smem;                     // panel: row r (0..127), column c at P[r * 17 + c]
     double *Li = smem + LDS_SMALL_PANEL;  // LU: inv(L11), row-major, stride 17
+    double *Rc = Li + 16 * BF_LD;         // Cholesky: 1 / diag(L11) of the panel
     // the diagonal factorization is the serial chain of the whole DAG: its
     // waves issue ahead of the GEMM waves sharing their SIMDs
     __builtin_amdgcn_s_setprio(3);
@@ -1111,16 +1113,21 @@ __device__ __attribute__((always_inline)) inline void factor_small_blk(const Uni
             {
                 const int r = p0 + lane;
                 const bool in = r < SMALL_MAX;
+                // lanes past the tile read the last row (their results are
+                // dropped): no branch around the loads
+                const double *src = P + (in ? r : SMALL_MAX - 1) * BF_LD;
                 double x[16];
 #pragma unroll
                 for (int c = 0; c < 16; c++)
-                    x[c] = in ? P[r * BF_LD + c] : 0.0;
+                    x[c] = src[c];
+                // the first failing column of the panel, uniform; reported
+                // once after the sweep instead of a branch in every column
+                int bad = 16;
 #pragma unroll
                 for (int c = 0; c < 16; c++) {
                     const double piv = readlane_d(x[c], c);
                     if (!CHOL) {
-                        if (lane == 0 && piv == 0.0)
-                            report_info(u, p0 + c);
+                        bad = min(bad, piv == 0.0 ? c : 16);
                         const double l = lane > c ? x[c] * rcp_nr(piv) : 0.0;
                         if (lane > c)
                             x[c] = l;
@@ -1129,19 +1136,20 @@ __device__ __attribute__((always_inline)) inline void factor_small_blk(const Uni
                         for (int k = c + 1; k < 16; k++)
                             x[k] = fma(-l, readlane_d(x[k], c), x[k]);
                     } else {
-                        if (lane == 0 && !(piv > 0.0))
-                            report_info(u, p0 + c);
+                        bad = min(bad, piv > 0.0 ? 16 : c);
                         const double rd = rsq_nr(piv);
-                        if (lane > c)
-                            x[c] *= rd;
-                        else if (lane == c)
-                            x[c] = piv * rd;
-                        const double l = lane > c ? x[c] : 0.0;
+                        Rc[c] = rd;
+                        // every lane, no masks: lane c gets piv * rd, the
+                        // lanes above it only change the diagonal block's
+                        // upper part, which nothing reads
+                        x[c] *= rd;
 #pragma unroll
                         for (int k = c + 1; k < 16; k++)  // L[p0 + k][c] is lane k's
-                            x[k] = fma(-l, readlane_d(x[c], k), x[k]);
+                            x[k] = fma(-x[c], readlane_d(x[c], k), x[k]);
                     }
                 }
+                if (bad < 16 && lane == 0)
+                    report_info(u, p0 + bad);
 #pragma unroll
                 for (int c = 0; c < 16; c++)
                     if (in)
@@ -1153,20 +1161,21 @@ __device__ __attribute__((always_inline)) inline void factor_small_blk(const Uni
             if (p0 + 64 < SMALL_MAX) {
                 const int r = p0 + 64 + lane;
                 const bool in = r < SMALL_MAX;
+                const double *src = P + (in ? r : SMALL_MAX - 1) * BF_LD;
                 double y[16];
 #pragma unroll
                 for (int c = 0; c < 16; c++)
-                    y[c] = in ? P[r * BF_LD + c] : 0.0;
+                    y[c] = src[c];
 #pragma unroll
                 for (int c = 0; c < 16; c++) {
                     const double *dr = P + (p0 + c) * BF_LD;  // LU: row c of U11
                     if (!CHOL) {
-                        y[c] *= rcp_nr(dr[c]);
+                        y[c] *= rcp_nr(dr[c]);  // the multiplier of sweep 1's rows
 #pragma unroll
                         for (int k = c + 1; k < 16; k++)
                             y[k] = fma(-y[c], dr[k], y[k]);
                     } else {
-                        y[c] *= rcp_nr(dr[c]);
+                        y[c] *= Rc[c];  // 1 / L11[c][c], as sweep 1 scaled its rows
 #pragma unroll
                         for (int k = c + 1; k < 16; k++)  // L11[k][c]
                             y[k] = fma(-y[c], P[(p0 + k) * BF_LD + c], y[k]);
