@@ -117,6 +117,10 @@ struct Check {
     std::atomic<int32_t> task_max{0};
     std::atomic<int32_t> tasks_active{0};  // tasks with a workgroup running one of their items
     std::atomic<int32_t> tasks_max{0};
+    // one task's count reaching 0 and the active-task count following it are
+    // one step: otherwise a workgroup of the same task starting in between
+    // would count the task twice
+    std::mutex task_mu;
     int body_barriers = 2;
 };
 
@@ -147,6 +151,7 @@ struct HostWave {
                     chk.violations++;
             (*chk.runs)[item]++;
             if (task >= 0 && task < (int)chk.task_now->size()) {
+                std::lock_guard<std::mutex> g(chk.task_mu);
                 const int32_t now = ++(*chk.task_now)[task];
                 raise_max(chk.task_max, now);
                 if (now == 1)
@@ -156,9 +161,11 @@ struct HostWave {
         // the item body's own barriers
         for (int b = 0; b < chk.body_barriers; b++)
             barrier();
-        if (w == 0 && task >= 0 && task < (int)chk.task_now->size())
+        if (w == 0 && task >= 0 && task < (int)chk.task_now->size()) {
+            std::lock_guard<std::mutex> g(chk.task_mu);
             if (--(*chk.task_now)[task] == 0)
                 --chk.tasks_active;
+        }
     }
 };
 
