@@ -225,23 +225,38 @@ def main():
         # HBM bytes per k_batch_gemm launch: FETCH_SIZE + WRITE_SIZE from the
         # rocprofv3 --pmc passes summarised in profiles/ (scaled by the same
         # counters on a copy of known size); not measurable inside this run
-        traffic = None
+        traffic, per_bytes = None, {}
         tp = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tp):
             with open(tp) as f:
                 tj = json.load(f)
             if tj.get("kernel") == "k_batch_gemm" and n == 24000 and t == 24:
                 traffic = tj["bytes_per_launch"]
+                per_bytes = {k: v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"]
+                             for k, v in tj.get("per_kernel", {}).items()}
         out["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3),
                            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                            "kernel": "k_batch_gemm", "launches": g["launches"],
                            "avg_launch_ms": round(g["ms"] / g["launches"], 4),
                            "avg_launch_gflop": round(g["flops"] / g["launches"] * 1e-9, 3)}
-        out["roofline"]["other_kernels"] = {
-            name: {"launches": o["launches"], "ms": round(o["ms"], 3),
-                   "tflops": round(o["flops"] / (o["ms"] * 1e-3) * 1e-12, 3)}
-            for name, o in per.items() if name != "k_batch_gemm"}
+        # the panel solves and diagonal factorizations beside it; hbm_gbs =
+        # their HBM bytes per launch from the same counter passes as
+        # `traffic` / this run's live average launch time
+        pmc_name = {"k_batch_trsm_256": "k_batch_trsm_mid"}
+        others = {}
+        for name, o in per.items():
+            if name == "k_batch_gemm":
+                continue
+            e = {"launches": o["launches"], "ms": round(o["ms"], 3),
+                 "tflops": round(o["flops"] / (o["ms"] * 1e-3) * 1e-12, 3)}
+            nb = per_bytes.get(pmc_name.get(name, name))
+            if nb:
+                e["hbm_bytes_per_launch"] = nb
+                e["hbm_gbs"] = round(nb * o["launches"] / (o["ms"] * 1e-3) * 1e-9, 1)
+                e["hbm_frac"] = round(e["hbm_gbs"] / PEAK_HBM_GBS, 4)
+            others[name] = e
+        out["roofline"]["other_kernels"] = others
         for name, p in per.items():
             log(f"[bench] {name}: {p['launches']} launches, {p['ms']:.1f} ms, "
                 f"{p['flops'] / (p['ms'] * 1e-3) * 1e-12:.2f} TFLOP/s")
