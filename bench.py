@@ -225,7 +225,7 @@ def main():
         # HBM bytes per k_batch_gemm launch: FETCH_SIZE + WRITE_SIZE from the
         # rocprofv3 --pmc passes summarised in profiles/ (scaled by the same
         # counters on a copy of known size); not measurable inside this run
-        traffic, per_bytes = None, {}
+        traffic, per_bytes, per_util = None, {}, {}
         tp = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tp):
             with open(tp) as f:
@@ -234,12 +234,16 @@ def main():
                 traffic = tj["bytes_per_launch"]
                 per_bytes = {k: v["fetch_bytes_per_launch"] + v["write_bytes_per_launch"]
                              for k, v in tj.get("per_kernel", {}).items()}
+                per_util = {k: v["mfma_util"] for k, v in tj.get("per_kernel", {}).items()
+                            if "mfma_util" in v}
         out["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3),
                            "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                            "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                            "kernel": "k_batch_gemm", "launches": g["launches"],
                            "avg_launch_ms": round(g["ms"] / g["launches"], 4),
                            "avg_launch_gflop": round(g["flops"] / g["launches"] * 1e-9, 3)}
+        if "k_batch_gemm" in per_util:  # from the rocprofv3 --pmc pass, not this run
+            out["roofline"]["mfma_util_pmc"] = per_util["k_batch_gemm"]
         # the panel solves and diagonal factorizations beside it; hbm_gbs =
         # their HBM bytes per launch from the same counter passes as
         # `traffic` / this run's live average launch time
@@ -255,6 +259,8 @@ def main():
                 e["hbm_bytes_per_launch"] = nb
                 e["hbm_gbs"] = round(nb * o["launches"] / (o["ms"] * 1e-3) * 1e-9, 1)
                 e["hbm_frac"] = round(e["hbm_gbs"] / PEAK_HBM_GBS, 4)
+            if pmc_name.get(name, name) in per_util:
+                e["mfma_util_pmc"] = per_util[pmc_name.get(name, name)]
             others[name] = e
         out["roofline"]["other_kernels"] = others
         for name, p in per.items():
